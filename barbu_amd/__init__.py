@@ -1,0 +1,330 @@
+"""barbu_amd: the FTL hair-strand step of tcoppex/barbu on MI355X (gfx950).
+
+The compute path is libftl.so (HIP kernels + C ABI, include/ftl.h).  This
+package is a thin ctypes layer over it for Python callers and tests; it has no
+compute of its own and fails loudly when the library is missing.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import (FTL_ROOT_MODEL, FTL_ROOT_REST, FTL_ROOT_SKINNED, FTL_VELOCITY_DFTL,
+                   FTL_VELOCITY_SHIPPED, FTL_MAX_COLLIDERS, FTL_MAX_POINTS, FTL_WIND_GRID,
+                   FtlError, Params, check, lib)
+
+__all__ = ["Context", "Groom", "Skin", "Batch", "Params", "reference_params", "FtlError",
+           "sphere_groom", "reference_groom", "skinned_scene", "bones", "wind_table",
+           "FTL_ROOT_MODEL", "FTL_ROOT_REST", "FTL_ROOT_SKINNED", "FTL_VELOCITY_SHIPPED",
+           "FTL_VELOCITY_DFTL", "Hair"]
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def reference_params():
+    """The shipped shader's constants (ftl_params_reference)."""
+    p = Params()
+    lib().ftl_params_reference(C.byref(p))
+    return p
+
+
+def make_params(**kw):
+    """Reference parameters with fields overridden by keyword."""
+    p = reference_params()
+    for k, v in kw.items():
+        if k in ("gravity", "wind_scroll"):
+            for i in range(3):
+                getattr(p, k)[i] = float(v[i])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def wind_table(params):
+    """The wind lattice selected by params: float32 [G^3, 4]."""
+    G = FTL_WIND_GRID
+    out = np.zeros((G * G * G, 4), np.float32)
+    g = C.c_int(0)
+    check(lib().ftl_wind_table(C.byref(params), _ptr(out), C.byref(g)), "ftl_wind_table")
+    return out
+
+
+class Context:
+    """A HIP device bound to the library (ftl_init)."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        check(lib().ftl_init(int(device), C.byref(self._h)), "ftl_init")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def memory(self):
+        v = C.c_uint64(0)
+        check(lib().ftl_ctx_memory(self._h, C.byref(v)), "ftl_ctx_memory")
+        return v.value
+
+    def autotune(self, cache_path=None, force=False):
+        path = cache_path.encode() if cache_path else None
+        check(lib().ftl_autotune(self._h, path, 1 if force else 0), "ftl_autotune")
+
+    def lanes_for(self, npoints):
+        return lib().ftl_lanes_for(self._h, int(npoints))
+
+    def close(self):
+        if self._h:
+            check(lib().ftl_shutdown(self._h), "ftl_shutdown")
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Groom:
+    """One groom's device state (ftl_groom_*)."""
+
+    def __init__(self, ctx, pos, vel=None, rest_roots=None):
+        pos = _f32(pos)
+        n = pos.shape[0]
+        self.nstrands, self.npoints = int(pos.shape[0]), int(pos.shape[1])
+        pos = pos.reshape(self.nstrands, self.npoints, 4)
+        vel = None if vel is None else _f32(vel, pos.shape)
+        roots = None if rest_roots is None else _f32(rest_roots, (n, 4))
+        d = _ffi.GroomDesc(self.nstrands, self.npoints, pos.ctypes.data,
+                           None if vel is None else vel.ctypes.data,
+                           None if roots is None else roots.ctypes.data)
+        self._h = C.c_void_p()
+        self.ctx = ctx
+        check(lib().ftl_groom_create(ctx.handle, C.byref(d), C.byref(self._h)), "ftl_groom_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_params(self, params):
+        check(lib().ftl_groom_set_params(self._h, C.byref(params)), "ftl_groom_set_params")
+
+    def get_params(self):
+        p = Params()
+        check(lib().ftl_groom_get_params(self._h, C.byref(p)), "ftl_groom_get_params")
+        return p
+
+    def set_colliders(self, spheres):
+        s = _f32(spheres).reshape(-1, 4)
+        check(lib().ftl_groom_set_colliders(self._h, _ptr(s), int(s.shape[0])), "ftl_groom_set_colliders")
+
+    def set_root_transform(self, m):
+        """m: 4x4 matrix (row-major numpy, math convention); stored column-major."""
+        m = np.asarray(m, np.float32).reshape(4, 4)
+        cm = np.ascontiguousarray(m.T)
+        check(lib().ftl_groom_set_root_transform(self._h, _ptr(cm)), "ftl_groom_set_root_transform")
+
+    def set_root_mode(self, mode):
+        check(lib().ftl_groom_set_root_mode(self._h, int(mode)), "ftl_groom_set_root_mode")
+
+    def set_rest_roots(self, roots):
+        r = _f32(roots, (self.nstrands, 4))
+        check(lib().ftl_groom_set_rest_roots(self._h, _ptr(r)), "ftl_groom_set_rest_roots")
+
+    def set_time(self, t):
+        check(lib().ftl_groom_set_time(self._h, float(t)), "ftl_groom_set_time")
+
+    def set_lanes(self, lanes):
+        check(lib().ftl_groom_set_lanes(self._h, int(lanes)), "ftl_groom_set_lanes")
+
+    def step(self, dt, stream=None):
+        check(lib().ftl_groom_step(self._h, float(dt), stream), "ftl_groom_step")
+
+    def download(self):
+        pos = np.empty((self.nstrands, self.npoints, 4), np.float32)
+        vel = np.empty_like(pos)
+        check(lib().ftl_groom_download(self._h, _ptr(pos), _ptr(vel)), "ftl_groom_download")
+        return pos, vel
+
+    def upload(self, pos=None, vel=None):
+        shape = (self.nstrands, self.npoints, 4)
+        pos = None if pos is None else _f32(pos, shape)
+        vel = None if vel is None else _f32(vel, shape)
+        check(lib().ftl_groom_upload(self._h, _ptr(pos), _ptr(vel)), "ftl_groom_upload")
+
+    def device_buffers(self):
+        p, v = C.c_void_p(), C.c_void_p()
+        check(lib().ftl_groom_device_buffers(self._h, C.byref(p), C.byref(v)), "ftl_groom_device_buffers")
+        return p.value, v.value
+
+    def status(self):
+        nf = C.c_int(0)
+        check(lib().ftl_groom_status(self._h, C.byref(nf)), "ftl_groom_status")
+        return bool(nf.value)
+
+    def bind_skin(self, skin, tri, bary):
+        tri = np.ascontiguousarray(tri, np.int32)
+        bary = _f32(bary, (self.nstrands, 2))
+        check(lib().ftl_groom_bind_skin(self._h, skin.handle, _ptr(tri), _ptr(bary)), "ftl_groom_bind_skin")
+
+    def close(self):
+        if self._h:
+            lib().ftl_groom_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Skin:
+    """Dual-quaternion skinned mesh (ftl_skin_*)."""
+
+    def __init__(self, ctx, rest, joints, weights, tris, nbones):
+        self.rest = _f32(rest).reshape(-1, 4)
+        self.joints = np.ascontiguousarray(joints, np.int32).reshape(-1, 4)
+        self.weights = _f32(weights).reshape(-1, 4)
+        self.tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        self.nbones = int(nbones)
+        d = _ffi.SkinDesc(self.rest.shape[0], self.tris.shape[0], self.nbones, self.rest.ctypes.data,
+                          self.joints.ctypes.data, self.weights.ctypes.data, self.tris.ctypes.data)
+        self._h = C.c_void_p()
+        check(lib().ftl_skin_create(ctx.handle, C.byref(d), C.byref(self._h)), "ftl_skin_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_bones(self, dq):
+        dq = _f32(dq).reshape(-1, 8)
+        check(lib().ftl_skin_set_bones(self._h, _ptr(dq), int(dq.shape[0])), "ftl_skin_set_bones")
+
+    def update(self, stream=None):
+        check(lib().ftl_skin_update(self._h, stream), "ftl_skin_update")
+
+    def download(self):
+        out = np.empty((self.rest.shape[0], 4), np.float32)
+        check(lib().ftl_skin_download(self._h, _ptr(out)), "ftl_skin_download")
+        return out
+
+    def close(self):
+        if self._h:
+            lib().ftl_skin_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """A fixed set of grooms stepped as one graph-captured frame (ftl_batch_*)."""
+
+    def __init__(self, ctx, grooms):
+        self.grooms = list(grooms)
+        arr = (C.c_void_p * len(self.grooms))(*[g.handle.value for g in self.grooms])
+        self._h = C.c_void_p()
+        check(lib().ftl_batch_create(ctx.handle, arr, len(self.grooms), C.byref(self._h)), "ftl_batch_create")
+
+    def launch(self, dt, stream=None):
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (len(self.grooms),)))
+        check(lib().ftl_batch_launch(self._h, _ptr(dt), stream), "ftl_batch_launch")
+
+    def info(self):
+        nt, gr = C.c_int(0), C.c_int(0)
+        check(lib().ftl_batch_info(self._h, C.byref(nt), C.byref(gr)), "ftl_batch_info")
+        return nt.value, gr.value
+
+    def close(self):
+        if self._h:
+            lib().ftl_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_step(ctx, grooms, dt, stream=None):
+    """One persistent launch over grooms (ftl_batch_step)."""
+    arr = (C.c_void_p * len(grooms))(*[g.handle.value for g in grooms])
+    dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (len(grooms),)))
+    check(lib().ftl_batch_step(ctx.handle, arr, len(grooms), _ptr(dt), stream), "ftl_batch_step")
+
+
+# --- scenario generators (CPU) ----------------------------------------------
+
+def sphere_groom(nstrands, npoints, center=(0, 0, 0), radius=1.0, length=0.2, jitter=0.05, seed=0,
+                 coherent=True):
+    """Strands on a sphere (ftl_scn_sphere_groom): (pos, vel, rest_roots)."""
+    d = _ffi.ScnSphere(int(nstrands), int(npoints), (C.c_float * 3)(*center), float(radius), float(length),
+                       float(jitter), int(seed), 1 if coherent else 0)
+    pos = np.empty((nstrands, npoints, 4), np.float32)
+    vel = np.empty_like(pos)
+    roots = np.empty((nstrands, 4), np.float32)
+    check(lib().ftl_scn_sphere_groom(C.byref(d), _ptr(pos), _ptr(vel), _ptr(roots)), "ftl_scn_sphere_groom")
+    return pos, vel, roots
+
+
+def reference_groom(roots, normals, npoints=4, maxlength=0.5, seed=0):
+    """Hair::init_simulation with a seeded stream (ftl_scn_reference_groom)."""
+    roots = _f32(roots).reshape(-1, 3)
+    normals = _f32(normals).reshape(-1, 3)
+    n = roots.shape[0]
+    pos = np.empty((n, npoints, 4), np.float32)
+    vel = np.empty_like(pos)
+    check(lib().ftl_scn_reference_groom(n, _ptr(roots), _ptr(normals), int(npoints), float(maxlength), int(seed),
+                                        _ptr(pos), _ptr(vel)), "ftl_scn_reference_groom")
+    return pos, vel
+
+
+def _skinned_desc(nstrands, npoints, ntris, nbones, radius, length, jitter, max_angle, seed):
+    return _ffi.ScnSkinned(int(nstrands), int(npoints), int(ntris), int(nbones), float(radius), float(length),
+                           float(jitter), float(max_angle), int(seed))
+
+
+def skinned_scene(nstrands, npoints, ntris=100_000, nbones=64, radius=1.0, length=0.2, jitter=0.05,
+                  max_angle=np.deg2rad(30.0), seed=0):
+    """Skinned emitter scene: dict with mesh arrays, binding and groom state."""
+    d = _skinned_desc(nstrands, npoints, ntris, nbones, radius, length, jitter, max_angle, seed)
+    nv, nt = C.c_int(0), C.c_int(0)
+    check(lib().ftl_scn_skinned_sizes(C.byref(d), C.byref(nv), C.byref(nt)), "ftl_scn_skinned_sizes")
+    rest = np.empty((nv.value, 4), np.float32)
+    joints = np.empty((nv.value, 4), np.int32)
+    weights = np.empty((nv.value, 4), np.float32)
+    tris = np.empty((nt.value, 3), np.int32)
+    check(lib().ftl_scn_skinned_mesh(C.byref(d), _ptr(rest), _ptr(joints), _ptr(weights), _ptr(tris)),
+          "ftl_scn_skinned_mesh")
+    tri = np.empty(nstrands, np.int32)
+    bary = np.empty((nstrands, 2), np.float32)
+    pos = np.empty((nstrands, npoints, 4), np.float32)
+    vel = np.empty_like(pos)
+    check(lib().ftl_scn_skinned_groom(C.byref(d), _ptr(rest), _ptr(tris), _ptr(tri), _ptr(bary), _ptr(pos),
+                                      _ptr(vel)), "ftl_scn_skinned_groom")
+    return dict(desc=d, rest=rest, joints=joints, weights=weights, tris=tris, tri=tri, bary=bary, pos=pos,
+                vel=vel, nbones=nbones)
+
+
+def bones(scene, t):
+    """Bone dual quaternions of a skinned scene at time t: float32 [nbones, 8]."""
+    d = scene["desc"]
+    dq = np.empty((d.nbones, 8), np.float32)
+    check(lib().ftl_scn_bones(C.byref(d), float(t), _ptr(dq)), "ftl_scn_bones")
+    return dq
+
+
+from .hair import Hair  # noqa: E402

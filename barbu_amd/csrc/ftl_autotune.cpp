@@ -1,0 +1,140 @@
+// Launch-shape autotuner: for each point count, times the exact launch
+// classes (lanes per strand 1, 2, 4) on a synthetic groom and keeps the
+// fastest.  Results do not depend on the choice (every class performs the
+// same IEEE operations per point); only the time does.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ftl.h"
+#include "ftl_classes.h"
+#include "ftl_internal.h"
+#include "ftl_launch.h"
+#include "ftl_scenario.h"
+
+static const int kTunedPoints[] = {4, 8, 16, 32, 64};
+
+static bool load_cache(ftl_ctx* ctx, const char* path) {
+  std::ifstream f(path);
+  if (!f) return false;
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string s = ss.str();
+  if (s.find("\"arch\": \"gfx950\"") == std::string::npos) return false;
+  char key[32];
+  std::snprintf(key, sizeof key, "\"ncu\": %d,", ctx->ncu);
+  if (s.find(key) == std::string::npos) return false;
+  const size_t at = s.find("\"lanes\"");
+  if (at == std::string::npos) return false;
+  int found = 0;
+  for (int n : kTunedPoints) {
+    std::snprintf(key, sizeof key, "\"%d\": ", n);
+    const size_t k = s.find(key, at);
+    if (k == std::string::npos) continue;
+    const int g = std::atoi(s.c_str() + k + std::strlen(key));
+    if (g == 1 || g == 2 || g == 4) {
+      ctx->lanes[n] = g;
+      ++found;
+    }
+  }
+  return found > 0;
+}
+
+static void save_cache(ftl_ctx* ctx, const char* path, const std::vector<std::string>& notes) {
+  std::ofstream f(path);
+  if (!f) return;
+  f << "{\n  \"arch\": \"gfx950\",\n  \"ncu\": " << ctx->ncu << ",\n  \"lanes\": {";
+  bool first = true;
+  for (int n : kTunedPoints) {
+    if (!ctx->lanes[n]) continue;
+    f << (first ? "" : ",") << "\n    \"" << n << "\": " << ctx->lanes[n];
+    first = false;
+  }
+  f << "\n  },\n  \"timings_us\": [";
+  for (size_t i = 0; i < notes.size(); ++i) f << (i ? ",\n    " : "\n    ") << notes[i];
+  f << "\n  ]\n}\n";
+}
+
+extern "C" ftl_status ftl_autotune(ftl_ctx* ctx, const char* cache_path, int force) {
+  if (!ctx) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  if (cache_path && !force && load_cache(ctx, cache_path)) return FTL_OK;
+  hipError_t e;
+  if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
+  hipEvent_t t0, t1;
+  if ((e = hipEventCreate(&t0)) != hipSuccess) return ftl_hip_fail(e, "hipEventCreate");
+  if ((e = hipEventCreate(&t1)) != hipSuccess) {
+    (void)hipEventDestroy(t0);
+    return ftl_hip_fail(e, "hipEventCreate");
+  }
+  std::vector<std::string> notes;
+  ftl_status st = FTL_OK;
+  for (int N : kTunedPoints) {
+    const int S = (1 << 22) / N;  // about 4M control points
+    ftl_scn_sphere sd{};
+    sd.nstrands = S;
+    sd.npoints = N;
+    sd.radius = 1.0f;
+    sd.length = 0.2f;
+    sd.jitter = 0.05f;
+    sd.seed = 12345;
+    sd.coherent = 1;
+    std::vector<float> pos((size_t)S * N * 4), vel((size_t)S * N * 4);
+    if ((st = ftl_scn_sphere_groom(&sd, pos.data(), vel.data(), nullptr)) != FTL_OK) break;
+    ftl_groom_desc gd{S, N, pos.data(), vel.data(), nullptr};
+    ftl_groom* g = nullptr;
+    if ((st = ftl_groom_create(ctx, &gd, &g)) != FTL_OK) break;
+    ftl_params p;
+    ftl_params_reference(&p);
+    p.force_scale = 1.0f;
+    p.length_scale = 1.0f;
+    p.damping = 0.9f;
+    ftl_groom_set_params(g, &p);
+    const float sph[8] = {0.f, 0.f, 0.f, 1.f, 0.f, 1.2f, 0.f, 0.3f};
+    ftl_groom_set_colliders(g, sph, 2);
+    int best_g = 0;
+    float best_t = 1e30f;
+    for (int G = 1; G <= 4; G *= 2) {
+      int cls = -1;
+      for (int c = 0; c < FTL_NCLASSES; ++c)
+        if (!kFtlClasses[c].guarded && kFtlClasses[c].g == G && kFtlClasses[c].m * G == N &&
+            kFtlClasses[c].m <= 32)
+          cls = c;
+      if (cls < 0) continue;
+      if ((st = ftl_groom_upload(g, pos.data(), vel.data())) != FTL_OK) break;
+      FtlStep A;
+      if ((st = ftl_make_step(g, 1.0f / 60.0f, &A)) != FTL_OK) break;
+      for (int w = 0; w < 2; ++w) ftl_launch_plain(cls, A, nullptr);
+      const int reps = 8;
+      (void)hipEventRecord(t0, nullptr);
+      for (int r = 0; r < reps; ++r) ftl_launch_plain(cls, A, nullptr);
+      (void)hipEventRecord(t1, nullptr);
+      if ((e = hipEventSynchronize(t1)) != hipSuccess) {
+        st = ftl_hip_fail(e, "autotune timing");
+        break;
+      }
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, t0, t1);
+      const float us = 1000.0f * ms / reps;
+      char note[96];
+      std::snprintf(note, sizeof note, "{\"npoints\": %d, \"lanes\": %d, \"us\": %.2f}", N, G, us);
+      notes.push_back(note);
+      if (us < best_t) {
+        best_t = us;
+        best_g = G;
+      }
+    }
+    ftl_groom_destroy(g);
+    if (st != FTL_OK) break;
+    if (best_g) ctx->lanes[N] = best_g;
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (st != FTL_OK) return st;
+  if (cache_path) save_cache(ctx, cache_path, notes);
+  return FTL_OK;
+}

@@ -1,0 +1,144 @@
+// Device-side definitions shared by the FTL kernels: the per-groom step
+// descriptor and the FP32 evaluation order of every formula of
+// src/shaders/hair/01_simulation/cs_simulation.glsl.
+//
+// Every compound expression is spelled out as the IEEE operations it performs
+// (mul, add, explicit fma) and the library is compiled with
+// -ffp-contract=off, so the compiler can neither fuse nor reassociate.  The
+// result of a step is therefore a fixed function of its inputs, independent of
+// the launch shape (lanes per strand, tile size, batching).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FTL_WAVE 64
+#define FTL_NCOL 4
+
+// One groom's step, by value in kernel arguments (plain launch) or in a device
+// table (batched launch).  Mirrors the SSBO bindings and uniforms of
+// cs_simulation.glsl:7-43 plus the constants it hard-codes.
+struct FtlStep {
+  float4* pos;                  // read_positions / write_positions (in place)
+  float4* vel;                  // read_velocities / write_velocities (in place)
+  const float4* roots;          // FTL_ROOT_REST: rest root per strand
+  const int4* bind;             // FTL_ROOT_SKINNED: {triangle, w1 bits, w2 bits, 0}
+  const int4* tris;             // FTL_ROOT_SKINNED: {a, b, c, 0} per triangle
+  const float4* skinned;        // FTL_ROOT_SKINNED: skinned vertices
+  const float4* wind;           // wind lattice, float4 [grid^3], or null
+  unsigned* status;             // bit 0: a strand tip became non-finite
+  int nstrands, npoints, iterations, vmode;
+  int rmode, ncol, model_ident, wind_grid;
+  float dt, dt2, inv_dt, sigma;
+  float delta, maxv, maxv2, wind_freq;
+  float force[4];               // kForceCoeff * gravity (w unused)
+  float sph[FTL_NCOL][4];       // collider centre xyz, radius
+  float r2[FTL_NCOL];           // radius * radius
+  float model[12];              // row-major 3x4 root transform
+  float wind_off[4];            // lattice offset of this step (w unused)
+};
+
+// Work unit of a batched launch: 64 lanes of one groom.
+struct FtlTile {
+  int groom;    // index into the FtlStep table
+  int strand0;  // first strand of the tile
+  int cls;      // launch class (points per lane, lanes per strand)
+  int cost;     // relative cost, for ordering
+};
+
+namespace ftl_dev {
+
+__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 mul3(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+// fma(s, d, b) per component
+__device__ __forceinline__ float3 fma3(float s, float3 d, float3 b) {
+  return f3(__builtin_fmaf(s, d.x, b.x), __builtin_fmaf(s, d.y, b.y), __builtin_fmaf(s, d.z, b.z));
+}
+// dot(a, b) = fma(a.z, b.z, fma(a.y, b.y, a.x * b.x))
+__device__ __forceinline__ float dot3(float3 a, float3 b) {
+  return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, a.x * b.x));
+}
+
+// inversesqrt: integer seed, then three Newton steps y <- y * (1.5 - (x/2) y^2).
+// Bit-reproducible (plain IEEE operations), <= 2 ulp.
+__device__ __forceinline__ float rsqrt_c(float x) {
+  unsigned i = __float_as_uint(x);
+  i = 0x5f3759dfu - (i >> 1);
+  float y = __uint_as_float(i);
+  const float xh = 0.5f * x;
+  y = y * __builtin_fmaf(-xh, y * y, 1.5f);
+  y = y * __builtin_fmaf(-xh, y * y, 1.5f);
+  y = y * __builtin_fmaf(-xh, y * y, 1.5f);
+  return y;
+}
+
+// Root transform: row c = fma(m_c0, x, fma(m_c1, y, fma(m_c2, z, t_c))).
+__device__ __forceinline__ float3 transform(const FtlStep& A, float3 x) {
+  const float* m = A.model;
+  return f3(__builtin_fmaf(m[0], x.x, __builtin_fmaf(m[1], x.y, __builtin_fmaf(m[2], x.z, m[3]))),
+            __builtin_fmaf(m[4], x.x, __builtin_fmaf(m[5], x.y, __builtin_fmaf(m[6], x.z, m[7]))),
+            __builtin_fmaf(m[8], x.x, __builtin_fmaf(m[9], x.y, __builtin_fmaf(m[10], x.z, m[11]))));
+}
+
+__device__ __forceinline__ float lerp_c(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
+
+// Wind acceleration at x: trilinear value noise over a periodic lattice with
+// smoothstep weights (the force hook of CalculateForces, cs_simulation.glsl:70).
+__device__ __forceinline__ float3 wind_at(const FtlStep& A, float3 x) {
+  const int G = A.wind_grid, msk = G - 1;
+  const float u0 = __builtin_fmaf(x.x, A.wind_freq, A.wind_off[0]);
+  const float u1 = __builtin_fmaf(x.y, A.wind_freq, A.wind_off[1]);
+  const float u2 = __builtin_fmaf(x.z, A.wind_freq, A.wind_off[2]);
+  const float l0 = floorf(u0), l1 = floorf(u1), l2 = floorf(u2);
+  const float f0 = u0 - l0, f1 = u1 - l1, f2 = u2 - l2;
+  const int i0 = (int)l0, i1 = (int)l1, i2 = (int)l2;
+  const int x0 = i0 & msk, x1 = (i0 + 1) & msk;
+  const int y0 = i1 & msk, y1 = (i1 + 1) & msk;
+  const int z0 = i2 & msk, z1 = (i2 + 1) & msk;
+  const float s0 = (f0 * f0) * __builtin_fmaf(-2.0f, f0, 3.0f);
+  const float s1 = (f1 * f1) * __builtin_fmaf(-2.0f, f1, 3.0f);
+  const float s2 = (f2 * f2) * __builtin_fmaf(-2.0f, f2, 3.0f);
+  const float4* T = A.wind;
+  const float4 v000 = T[(z0 * G + y0) * G + x0], v001 = T[(z0 * G + y0) * G + x1];
+  const float4 v010 = T[(z0 * G + y1) * G + x0], v011 = T[(z0 * G + y1) * G + x1];
+  const float4 v100 = T[(z1 * G + y0) * G + x0], v101 = T[(z1 * G + y0) * G + x1];
+  const float4 v110 = T[(z1 * G + y1) * G + x0], v111 = T[(z1 * G + y1) * G + x1];
+  float3 r;
+#define FTL_TRI(c)                                                      \
+  {                                                                     \
+    const float c00 = lerp_c(v000.c, v001.c, s0);                       \
+    const float c10 = lerp_c(v010.c, v011.c, s0);                       \
+    const float c01 = lerp_c(v100.c, v101.c, s0);                       \
+    const float c11 = lerp_c(v110.c, v111.c, s0);                       \
+    r.c = lerp_c(lerp_c(c00, c10, s1), lerp_c(c01, c11, s1), s2);       \
+  }
+  FTL_TRI(x) FTL_TRI(y) FTL_TRI(z)
+#undef FTL_TRI
+  return r;
+}
+
+// CollideSphere(+1, c_k, r_k, p) for every collider in order
+// (cs_simulation.glsl:129-153).  REFLECT: also reflect u on each hit; only the
+// last constraint iteration's velocity reaches the output.
+template <bool REFLECT>
+__device__ __forceinline__ void collide(const FtlStep& A, float3& p, float3& u) {
+  const int nc = A.ncol;
+#pragma unroll 1
+  for (int k = 0; k < nc; ++k) {
+    const float3 c = f3(A.sph[k][0], A.sph[k][1], A.sph[k][2]);
+    const float3 w = sub3(p, c);
+    const float dp = dot3(w, w);
+    if (dp < A.r2[k]) {
+      const float is = rsqrt_c(dp);
+      const float3 n = mul3(w, is);
+      p = fma3(A.sph[k][3], n, c);
+      if (REFLECT) {
+        const float d2 = 2.0f * dot3(n, u);
+        u = fma3(-d2, n, u);
+      }
+    }
+  }
+}
+
+}  // namespace ftl_dev

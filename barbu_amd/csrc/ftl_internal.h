@@ -1,0 +1,71 @@
+// Internal host-side state of the C ABI objects.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "ftl.h"
+#include "ftl_device.h"
+
+struct ftl_ctx {
+  int device = 0;
+  int ncu = 0;
+  int lanes[FTL_MAX_POINTS + 1];  // tuned lanes per strand by point count (0 = default rule)
+  std::mutex mu;
+  uint64_t bytes = 0;  // device memory held by the context's objects
+  int live_objects = 0;
+  // scratch of ftl_batch_step
+  void* scratch_steps = nullptr;
+  size_t scratch_steps_bytes = 0;
+  void* scratch_tiles = nullptr;
+  size_t scratch_tiles_bytes = 0;
+  unsigned* scratch_counter = nullptr;
+};
+
+struct ftl_skin {
+  ftl_ctx* ctx = nullptr;
+  int nverts = 0, ntris = 0, nbones = 0;
+  void* d_rest = nullptr;
+  void* d_joints = nullptr;
+  void* d_weights = nullptr;
+  void* d_out = nullptr;
+  void* d_tris = nullptr;
+  void* d_dq = nullptr;
+  std::vector<float> h_dq;
+  bool dirty = false;
+};
+
+struct ftl_groom {
+  ftl_ctx* ctx = nullptr;
+  int nstrands = 0, npoints = 0;
+  void* d_pos = nullptr;
+  void* d_vel = nullptr;
+  void* d_roots = nullptr;
+  void* d_bind = nullptr;
+  unsigned* d_status = nullptr;
+  void* d_wind = nullptr;
+  uint64_t wind_seed = 0;
+  float wind_amp = 0.0f;
+  ftl_params params;
+  int ncol = 1;
+  float sph[FTL_MAX_COLLIDERS][4];
+  float model[16];
+  int model_ident = 1;
+  int root_mode = FTL_ROOT_MODEL;
+  double time = 0.0;
+  ftl_skin* skin = nullptr;
+  int lanes = 0;
+};
+
+ftl_status ftl_fail(ftl_status st, const char* fmt, ...);
+ftl_status ftl_hip_fail(hipError_t e, const char* what);
+ftl_status ftl_dev_alloc(ftl_ctx* ctx, void** p, size_t bytes, const char* what);
+void ftl_dev_free(ftl_ctx* ctx, void* p, size_t bytes);
+// Fills the kernel descriptor of one step of g.
+ftl_status ftl_make_step(ftl_groom* g, float dt, FtlStep* A);
+// Launch class for npoints: requested lanes (0 = tuned/default), at most mmax
+// points per lane.  -1 if none.
+int ftl_class_for(ftl_ctx* ctx, int npoints, int lanes, int mmax);

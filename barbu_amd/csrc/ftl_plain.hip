@@ -1,0 +1,49 @@
+// Plain (one groom) FTL step kernel for one launch class, selected at build
+// time with -DFTL_CLASS_ID=<id>: one wave64 workgroup per tile of strands.
+#include "ftl_classes.h"
+#include "ftl_launch.h"
+#include "ftl_tile.h"
+
+#ifndef FTL_CLASS_ID
+#error "build with -DFTL_CLASS_ID=<launch class>"
+#endif
+
+namespace {
+
+template <int ID>
+struct ClassOf;
+#define FTL_CO(id, M_, G_, GU_)              \
+  template <>                                \
+  struct ClassOf<id> {                       \
+    static constexpr int M = M_, G = G_;     \
+    static constexpr bool GU = GU_;          \
+  };
+FTL_CLASSES(FTL_CO)
+#undef FTL_CO
+
+using CL = ClassOf<FTL_CLASS_ID>;
+
+// Occupancy target: the waves per SIMD the class's register state allows
+// (3 x M VGPRs of positions + M of rest lengths per lane).
+constexpr int kWavesPerEu = CL::M <= 4 ? 8 : CL::M <= 8 ? 5 : CL::M <= 16 ? 3 : 2;
+
+__global__ __launch_bounds__(FTL_WAVE) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
+void ftl_plain_kernel(FtlStep A) {
+  constexpr int C = ftl_tile_impl::Tile<CL::M, CL::G, CL::GU>::C;
+  __shared__ float4 stage[FTL_WAVE * C];
+  __shared__ float4 u0s[FTL_WAVE];
+  ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU>(A, blockIdx.x * (FTL_WAVE / CL::G), stage, u0s);
+}
+
+}  // namespace
+
+#define FTL_PASTE2(a, b) a##b
+#define FTL_PASTE(a, b) FTL_PASTE2(a, b)
+
+hipError_t FTL_PASTE(ftl_launch_plain_, FTL_CLASS_ID)(const FtlStep& A, hipStream_t stream) {
+  constexpr int spt = FTL_WAVE / CL::G;
+  const int nblocks = (A.nstrands + spt - 1) / spt;
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ftl_plain_kernel, dim3(nblocks), dim3(FTL_WAVE), 0, stream, A);
+  return hipGetLastError();
+}

@@ -1,0 +1,268 @@
+// Tile function of the FTL step kernels (included by the per-class kernel
+// translation units).
+//
+// Semantics: src/shaders/hair/01_simulation/cs_simulation.glsl (tcoppex/barbu)
+//   predict / root          :179-193
+//   SatisfyConstraints      :155-161 (K x {DistanceConstraint; CollisionConstraint})
+//   DistanceConstraint      :93-127  (serial FTL sweep + damped velocity shift)
+//   CollideSphere           :129-147
+//   Eq. 9 / VelocityControl :79-84, :200-204 (DFTL mode)
+//
+// Mapping (the reference runs one 4-lane workgroup per strand with a serial
+// sweep on lane 0, :88-127):
+//   * A tile is one wave64.  Each strand is owned by G consecutive lanes
+//     (G = 1, 2 or 4); lane g holds control points [g*M, g*M + M) in VGPRs for
+//     the whole step (M = npoints / G).
+//   * Load: the tile's strands are contiguous in HBM; they are read in chunks of
+//     C points per lane with fully coalesced float4 loads, integrated (predict /
+//     root) in that layout, and transposed through an XOR-swizzled LDS buffer.
+//   * Constraints: each lane sweeps its own points.  With G > 1 the lanes of a
+//     strand form a pipeline across iterations: lane g runs iteration t - g at
+//     time step t and takes its leader point (lane g-1's last point, before
+//     collisions) through a cross-lane shuffle.  K iterations take K + G - 1
+//     time steps.
+//   * Only the last iteration's velocity reaches the output (every iteration
+//     overwrites u_i for i >= 1 before reading it), so velocity work, the
+//     damped shift and the reflections are done in the last iteration only,
+//     and each point is written to HBM as soon as it is final.
+//   * In place: a tile owns its strands, so no second buffer and no copy-back.
+
+#pragma once
+
+#include "ftl_device.h"
+#ifndef FTL_WIND_ON
+#define FTL_WIND_ON 1
+#endif
+
+using namespace ftl_dev;
+
+namespace ftl_tile_impl {
+
+enum { kVelShipped = 0, kVelDftl = 1 };
+enum { kRootModel = 0, kRootRest = 1, kRootSkinned = 2 };
+
+template <int M, int G, bool GUARD>
+struct Tile {
+  static constexpr int C = (M < 8) ? M : 8;  // points per lane per load chunk
+  static constexpr int SPT = FTL_WAVE / G;   // strands per tile
+};
+
+// Stores one final control point through the lane's base pointers (constant
+// offsets, so the stores use immediate offsets and no per-point address).
+__device__ __forceinline__ void store_point(const FtlStep& A, float4* __restrict__ pp,
+                                            float4* __restrict__ vp, int j, float3 p, float w, float3 u) {
+  float3 v = u;
+  if (A.vmode == kVelDftl) {
+    // v = ((p - x) - u) / dt, then VelocityControl (:79-84).  x is still in
+    // memory: this point has not been written yet.
+    const float4 x = pp[j];
+    v = f3(((p.x - x.x) - u.x) * A.inv_dt, ((p.y - x.y) - u.y) * A.inv_dt,
+           ((p.z - x.z) - u.z) * A.inv_dt);
+    const float dp = dot3(v, v);
+    if (dp > A.maxv2) v = mul3(v, A.maxv * rsqrt_c(dp));
+  }
+  pp[j] = make_float4(p.x, p.y, p.z, w);
+  vp[j] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+__device__ __forceinline__ float3 root_source(const FtlStep& A, int s, float4 x) {
+  if (A.rmode == kRootRest) {
+    const float4 r = A.roots[s];
+    return f3(r.x, r.y, r.z);
+  }
+  if (A.rmode == kRootSkinned) {
+    const int4 b = A.bind[s];
+    const int4 t = A.tris[b.x];
+    const float w1 = __int_as_float(b.y), w2 = __int_as_float(b.z);
+    const float w0 = (1.0f - w1) - w2;
+    const float4 a = A.skinned[t.x], bb = A.skinned[t.y], c = A.skinned[t.z];
+    return f3(__builtin_fmaf(w2, c.x, __builtin_fmaf(w1, bb.x, w0 * a.x)),
+              __builtin_fmaf(w2, c.y, __builtin_fmaf(w1, bb.y, w0 * a.y)),
+              __builtin_fmaf(w2, c.z, __builtin_fmaf(w1, bb.z, w0 * a.z)));
+  }
+  return f3(x.x, x.y, x.z);
+}
+
+// One sweep of DistanceConstraint over this lane's points, fused with the
+// collisions of the points it has passed.  FINAL: last iteration, which also
+// forms the velocities and writes every point but the lane's last.
+template <int M, int G, bool GUARD, bool FINAL>
+__device__ __forceinline__ void sweep(const FtlStep& A, float3 (&p)[M], const float (&l)[M], float3 lead,
+                                      int g, int nl, bool valid, float4* __restrict__ pp,
+                                      float4* __restrict__ vp, float3 u0,
+                                      float3& last_pre, float3& cfirst, float3& clast) {
+  // Opaque per sweep: keeps the compiler from hoisting sigma * l[j] out of the
+  // iteration loop into M extra registers (one multiply per point instead).
+  float sg = A.sigma;
+  asm volatile("" : "+v"(sg));
+  const float dl = A.delta;
+  constexpr int J0 = (G == 1) ? 1 : 0;  // with G == 1 slot 0 is the root
+#pragma unroll
+  for (int j = J0; j < M; ++j) {
+    if (GUARD && j >= nl) break;
+    const float3 q = p[j];
+    const float3 d = sub3(q, lead);
+    const float s = (sg * l[j]) * rsqrt_c(dot3(d, d));
+    float3 pn = fma3(s, d, lead);
+    if (G > 1 && j == 0) pn = (g == 0) ? q : pn;  // lane 0's slot 0 is the root
+    p[j] = pn;
+    lead = pn;
+    if (FINAL) {
+      const float3 c = sub3(pn, q);
+      if (G > 1 && j == 0) cfirst = c;
+      if (j >= 1) {
+        // Point j-1 is final: u = delta * c_j (root keeps its own velocity).
+        float3 pm = p[j - 1];
+        float3 u = mul3(c, dl);
+        if (j - 1 == 0 && (G == 1 || g == 0)) {
+          u = u0;
+        } else {
+          collide<true>(A, pm, u);
+        }
+        p[j - 1] = pm;
+        if (valid) store_point(A, pp, vp, j - 1, pm, l[j - 1], u);
+      }
+      if (GUARD ? (j == nl - 1) : (j == M - 1)) {
+        clast = c;
+        last_pre = pn;
+      }
+    } else {
+      if (j >= 1 && !(j - 1 == 0 && (G == 1 || g == 0))) {
+        float3 dummy;
+        collide<false>(A, p[j - 1], dummy);
+      }
+      if (GUARD ? (j == nl - 1) : (j == M - 1)) {
+        last_pre = pn;
+        float3 dummy;
+        collide<false>(A, p[j], dummy);
+      }
+    }
+  }
+}
+
+template <int M, int G, bool GUARD>
+__device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* __restrict__ stage,
+                                         float4* __restrict__ u0s) {
+  constexpr int C = Tile<M, G, GUARD>::C;
+  const int lane = threadIdx.x & (FTL_WAVE - 1);
+  const int g = (G == 1) ? 0 : (lane % G);
+  const int N = GUARD ? A.npoints : M * G;
+  const int nl = GUARD ? A.npoints : M;  // points held by this lane (G == 1 when GUARD)
+  const int strand = strand0 + lane / G;
+  const bool valid = strand < A.nstrands;
+  const int slast = A.nstrands - 1;
+
+  float3 p[M];
+  float l[M];
+
+  // ---- load + integrate, through LDS ------------------------------------
+  // Loads are clamped into the groom (out-of-range lanes read valid data that
+  // is never stored), so all 2C loads of a chunk are issued back to back.
+  const float dt = A.dt, dt2 = A.dt2;
+  const float3 F0 = f3(A.force[0], A.force[1], A.force[2]);
+#pragma unroll
+  for (int c = 0; c < M / C; ++c) {
+    if (GUARD && c * C >= nl) break;
+    float4 xs[C], vs[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int q = k * FTL_WAVE + lane;
+      const int r = q / C;  // owner lane
+      const int s = min(strand0 + r / G, slast);
+      int cp = (r % G) * M + c * C + (q % C);
+      if (GUARD) cp = min(cp, N - 1);
+      const size_t idx = (size_t)s * N + cp;
+      xs[k] = A.pos[idx];
+      vs[k] = A.vel[idx];
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int q = k * FTL_WAVE + lane;
+      const int r = q / C;
+      const int j = q % C;
+      const int cp = (r % G) * M + c * C + j;
+      const float4 x = xs[k], v = vs[k];
+      float3 F = F0;
+      if (FTL_WIND_ON && A.wind_grid) {
+        const float3 w = wind_at(A, f3(x.x, x.y, x.z));
+        F = f3(F0.x + w.x, F0.y + w.y, F0.z + w.z);
+      }
+      float3 pp = f3(__builtin_fmaf(dt2, F.x, __builtin_fmaf(dt, v.x, x.x)),
+                     __builtin_fmaf(dt2, F.y, __builtin_fmaf(dt, v.y, x.y)),
+                     __builtin_fmaf(dt2, F.z, __builtin_fmaf(dt, v.z, x.z)));
+      if (cp == 0) {
+        const float3 src = root_source(A, min(strand0 + r / G, slast), x);
+        pp = A.model_ident ? src : transform(A, src);
+        const float3 u = sub3(pp, f3(x.x, x.y, x.z));
+        u0s[r] = make_float4(u.x, u.y, u.z, 0.0f);
+      }
+      stage[r * C + (j ^ (r & (C - 1)))] = make_float4(pp.x, pp.y, pp.z, x.w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const float4 t = stage[lane * C + (j ^ (lane & (C - 1)))];
+      p[c * C + j] = f3(t.x, t.y, t.z);
+      l[c * C + j] = t.w;
+    }
+    __syncthreads();
+  }
+  float3 u0 = f3(0.f, 0.f, 0.f);
+  if (g == 0) {
+    const float4 t = u0s[lane];
+    u0 = f3(t.x, t.y, t.z);
+  }
+
+  // ---- constraints ------------------------------------------------------
+  const size_t base = (size_t)(valid ? strand : 0) * N + (size_t)g * M;
+  float4* __restrict__ pp = A.pos + base;
+  float4* __restrict__ vp = A.vel + base;
+  const int K = A.iterations;
+  float3 last_pre = p[0];
+  float3 cfirst = f3(0.f, 0.f, 0.f), clast = f3(0.f, 0.f, 0.f);
+  if (G == 1) {
+    for (int it = 0; it < K - 1; ++it)
+      sweep<M, G, GUARD, false>(A, p, l, p[0], 0, nl, valid, pp, vp, u0, last_pre, cfirst, clast);
+    sweep<M, G, GUARD, true>(A, p, l, p[0], 0, nl, valid, pp, vp, u0, last_pre, cfirst, clast);
+  } else {
+    for (int t = 0; t < K + G - 1; ++t) {
+      const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
+                           __shfl_up(last_pre.z, 1, G));
+      const float3 lead = (g == 0) ? p[0] : li;
+      const int it = t - g;
+      if (it >= 0 && it < K) {
+        if (it < K - 1)
+          sweep<M, G, GUARD, false>(A, p, l, lead, g, nl, valid, pp, vp, u0, last_pre, cfirst, clast);
+        else
+          sweep<M, G, GUARD, true>(A, p, l, lead, g, nl, valid, pp, vp, u0, last_pre, cfirst, clast);
+      }
+    }
+  }
+
+  // ---- the lane's last point: its velocity needs the next lane's correction
+  float3 u = clast;
+  if (G > 1) {
+    const float3 cn = f3(__shfl_down(cfirst.x, 1, G), __shfl_down(cfirst.y, 1, G),
+                         __shfl_down(cfirst.z, 1, G));
+    if (g != G - 1) u = mul3(cn, A.delta);
+  }
+  bool bad = false;
+  // (GUARD: the last point is nl - 1, selected with uniform branches)
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    if (GUARD ? (j == nl - 1) : (j == M - 1)) {
+      float3 pm = p[j];
+      collide<true>(A, pm, u);
+      p[j] = pm;
+      if (valid) {
+        store_point(A, pp, vp, j, pm, l[j], u);
+        bad = !(isfinite(pm.x) && isfinite(pm.y) && isfinite(pm.z) && isfinite(u.x) &&
+                isfinite(u.y) && isfinite(u.z));
+      }
+    }
+  }
+  const unsigned long long b = __ballot(bad);
+  if (b != 0ull && lane == (int)(__ffsll((long long)b) - 1)) atomicOr(A.status, 1u);
+}
+
+}  // namespace

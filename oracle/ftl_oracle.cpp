@@ -1,0 +1,454 @@
+// FTL hair-strand step: CPU oracle.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls this
+// file.  The parity tests, the smoke check and the CPU-baseline leg of the
+// benchmark use it as the checker; it is never the thing measured as the
+// product, and the product never falls back to it.
+//
+// What it restates (paths relative to the tcoppex/barbu tree):
+//   src/shaders/hair/01_simulation/cs_simulation.glsl   (whole file)
+//     UnpackParticle/PackParticle        :53-66
+//     CalculateForces                    :70-77   (+ a wind hook: value noise)
+//     VelocityControl                    :79-84   (DFTL mode only)
+//     DistanceConstraint                 :93-127  (serial sweep + damped shift)
+//     CollideSphere / CollisionConstraint:129-153
+//     SatisfyConstraints                 :155-161
+//     main (predict / root / pack)       :170-208
+//   src/shaders/hair/interop.h:8-30        (float4 pos with w = rest length, float4 vel)
+//   src/fx/hair.cc:89-125                  (one update = one dispatch; the next
+//                                            step reads this step's output)
+//   src/shaders/shared/inc_skinning.glsl:22-82  (dual-quaternion blending, roots)
+//
+// The GLSL leaves rounding of compound expressions to the implementation:
+// without `precise` a compiler may fuse a*b+c and apply a scalar factor once.
+// The FP32 path below fixes ONE such permitted evaluation order, written out
+// operation by operation, so that any other implementation that performs the
+// same IEEE operations in the same order (the HIP kernels do) produces the same
+// bits.  `inversesqrt` is left to the implementation by GLSL (2 ulp); the FP32
+// path uses a bit-reproducible integer-seeded Newton iteration (<= 2 ulp,
+// checked exhaustively over one binade pair by the tests).  The FP64 path
+// evaluates the same formulas in double with 1/sqrt, and serves as the
+// accuracy reference for the FP32 path.
+//
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off, no fast-math).
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+
+// Mirrors the uniforms of cs_simulation.glsl plus the parameters the shipped
+// shader hard-codes (force coefficient, iteration count, damping).
+typedef struct oracle_params {
+  int   npoints;          // HAIR_MAX_PARTICLE_PER_STRAND (interop.h:8)
+  int   iterations;       // kNumContraintsIteration (cs_simulation.glsl:197)
+  int   velocity_mode;    // 0 = shipped (stored velocity = damped correction), 1 = DFTL Eq. 9 (:201-204)
+  int   root_mode;        // 0 = p0 = M * x0 (:190-191), 1 = p0 = M * roots[strand]
+  float dt;               // uTimeStep
+  float inv_dt;           // 1/dt, used by DFTL mode
+  float length_scale;     // uScaleFactor (sigma)
+  float damping;          // dftl_damp_scale (delta)
+  float max_velocity;     // uMaxVelocity
+  float force[3];         // kForceCoeff * gravity (already multiplied in float)
+  int   ncolliders;       // 0..4 spheres, applied in order (uBoundingSphere is one)
+  float spheres[4][4];    // xyz centre, w radius
+  float model[12];        // row-major 3x4 affine applied to roots
+  int   model_is_identity;
+  int   wind_grid;        // 0 = no wind, else lattice side (power of two)
+  float wind_frequency;   // lattice cells per metre
+  float wind_offset[3];   // lattice-space offset for this step
+  const float* wind_table;  // float4 [grid^3], index (z*grid + y)*grid + x
+} oracle_params;
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+struct V3 {
+  T x, y, z;
+};
+
+template <typename T>
+inline V3<T> sub(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+// dot(a, b) evaluated as fma(a.z, b.z, fma(a.y, b.y, a.x * b.x)).
+template <typename T>
+inline T dot3(V3<T> a, V3<T> b) {
+  return std::fma(a.z, b.z, std::fma(a.y, b.y, a.x * b.x));
+}
+
+// inversesqrt.  FP32: seed from the exponent/mantissa bits, then three Newton
+// steps y <- y * (1.5 - (x/2) * y^2).  FP64: 1/sqrt.
+inline float rsqrt_t(float x) {
+  uint32_t i;
+  std::memcpy(&i, &x, 4);
+  i = 0x5f3759dfu - (i >> 1);
+  float y;
+  std::memcpy(&y, &i, 4);
+  const float xh = 0.5f * x;
+  y = y * std::fma(-xh, y * y, 1.5f);
+  y = y * std::fma(-xh, y * y, 1.5f);
+  y = y * std::fma(-xh, y * y, 1.5f);
+  return y;
+}
+inline double rsqrt_t(double x) { return 1.0 / std::sqrt(x); }
+
+template <typename T>
+struct Particle {  // Particle_t (cs_simulation.glsl:47-51)
+  V3<T> position;
+  V3<T> velocity;
+  T restlength;
+};
+
+template <typename T>
+struct StepConsts {
+  T dt, dt2, inv_dt, scale, damp, maxv, maxv2;
+  V3<T> force;
+  int n, iters, vmode, rmode, ncol;
+  V3<T> c[4];
+  T r[4], r2[4];
+  T m[12];
+  bool ident;
+  int wgrid;
+  T wfreq;
+  V3<T> woff;
+  const float* wtab;
+};
+
+template <typename T>
+StepConsts<T> make_consts(const oracle_params* P) {
+  StepConsts<T> k{};
+  k.dt = (T)P->dt;
+  k.dt2 = k.dt * k.dt;
+  k.inv_dt = (T)P->inv_dt;
+  k.scale = (T)P->length_scale;
+  k.damp = (T)P->damping;
+  k.maxv = (T)P->max_velocity;
+  k.maxv2 = k.maxv * k.maxv;
+  k.force = {(T)P->force[0], (T)P->force[1], (T)P->force[2]};
+  k.n = P->npoints;
+  k.iters = P->iterations;
+  k.vmode = P->velocity_mode;
+  k.rmode = P->root_mode;
+  k.ncol = P->ncolliders < 0 ? 0 : (P->ncolliders > 4 ? 4 : P->ncolliders);
+  for (int s = 0; s < 4; ++s) {
+    k.c[s] = {(T)P->spheres[s][0], (T)P->spheres[s][1], (T)P->spheres[s][2]};
+    k.r[s] = (T)P->spheres[s][3];
+    k.r2[s] = k.r[s] * k.r[s];  // radius * radius (:133)
+  }
+  for (int i = 0; i < 12; ++i) k.m[i] = (T)P->model[i];
+  k.ident = P->model_is_identity != 0;
+  k.wgrid = P->wind_table ? P->wind_grid : 0;
+  k.wfreq = (T)P->wind_frequency;
+  k.woff = {(T)P->wind_offset[0], (T)P->wind_offset[1], (T)P->wind_offset[2]};
+  k.wtab = P->wind_table;
+  return k;
+}
+
+// Affine root transform p = M * (x, 1), row c evaluated as
+// fma(m_c0, x, fma(m_c1, y, fma(m_c2, z, t_c))).
+template <typename T>
+V3<T> transform(const StepConsts<T>& k, V3<T> x) {
+  if (k.ident) return x;
+  const T* m = k.m;
+  return {std::fma(m[0], x.x, std::fma(m[1], x.y, std::fma(m[2], x.z, m[3]))),
+          std::fma(m[4], x.x, std::fma(m[5], x.y, std::fma(m[6], x.z, m[7]))),
+          std::fma(m[8], x.x, std::fma(m[9], x.y, std::fma(m[10], x.z, m[11])))};
+}
+
+template <typename T>
+inline T lerp_t(T a, T b, T t) { return std::fma(t, b - a, a); }
+
+// Wind: trilinear value noise over a periodic lattice of random vectors, with
+// smoothstep weights.  The shipped shader has no wind; CalculateForces (:70-77)
+// takes the particle and ignores it, which is the hook this fills.
+template <typename T>
+V3<T> wind(const StepConsts<T>& k, V3<T> x) {
+  if (k.wgrid == 0) return {T(0), T(0), T(0)};
+  const int G = k.wgrid, msk = G - 1;
+  T u[3] = {std::fma(x.x, k.wfreq, k.woff.x), std::fma(x.y, k.wfreq, k.woff.y),
+            std::fma(x.z, k.wfreq, k.woff.z)};
+  int i0[3], i1[3];
+  T s[3];
+  for (int c = 0; c < 3; ++c) {
+    const T fl = std::floor(u[c]);
+    const T f = u[c] - fl;
+    const int ii = (int)fl;
+    i0[c] = ii & msk;
+    i1[c] = (ii + 1) & msk;
+    s[c] = (f * f) * std::fma(T(-2), f, T(3));
+  }
+  V3<T> v[2][2][2];  // [dz][dy][dx]
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        const int ix = dx ? i1[0] : i0[0], iy = dy ? i1[1] : i0[1], iz = dz ? i1[2] : i0[2];
+        const float* e = k.wtab + 4 * ((iz * G + iy) * G + ix);
+        v[dz][dy][dx] = {(T)e[0], (T)e[1], (T)e[2]};
+      }
+  V3<T> r;
+  T* out = &r.x;
+  for (int c = 0; c < 3; ++c) {
+    auto g = [&](int dz, int dy, int dx) { return (&v[dz][dy][dx].x)[c]; };
+    const T c00 = lerp_t(g(0, 0, 0), g(0, 0, 1), s[0]);
+    const T c10 = lerp_t(g(0, 1, 0), g(0, 1, 1), s[0]);
+    const T c01 = lerp_t(g(1, 0, 0), g(1, 0, 1), s[0]);
+    const T c11 = lerp_t(g(1, 1, 0), g(1, 1, 1), s[0]);
+    const T c0 = lerp_t(c00, c10, s[1]);
+    const T c1 = lerp_t(c01, c11, s[1]);
+    out[c] = lerp_t(c0, c1, s[2]);
+  }
+  return r;
+}
+
+// CollideSphere(+1, centre, radius, p)  (cs_simulation.glsl:129-139).
+template <typename T>
+void collide_sphere(const StepConsts<T>& k, int s, Particle<T>& p) {
+  const V3<T> pt = sub(p.position, k.c[s]);
+  const T dp = dot3(pt, pt);
+  if (dp < k.r2[s]) {
+    const T is = rsqrt_t(dp);
+    const V3<T> n = {pt.x * is, pt.y * is, pt.z * is};
+    // center + radius * n
+    p.position = {std::fma(k.r[s], n.x, k.c[s].x), std::fma(k.r[s], n.y, k.c[s].y),
+                  std::fma(k.r[s], n.z, k.c[s].z)};
+    // reflect(v, n) = v - 2 dot(n, v) n
+    const T d2 = T(2) * dot3(n, p.velocity);
+    p.velocity = {std::fma(-d2, n.x, p.velocity.x), std::fma(-d2, n.y, p.velocity.y),
+                  std::fma(-d2, n.z, p.velocity.z)};
+  }
+}
+
+// One strand, one step: main() of cs_simulation.glsl run by all N invocations
+// of one workgroup, with DistanceConstraint's lane-0 loops over the shared array.
+template <typename T, typename S>
+void step_strand(const StepConsts<T>& k, S* pos, S* vel, const S* root) {
+  const int N = k.n;
+  Particle<T> P[64];
+  V3<T> last[64];
+  // UnpackParticle (:53-61)
+  for (int i = 0; i < N; ++i) {
+    P[i].position = {(T)pos[4 * i + 0], (T)pos[4 * i + 1], (T)pos[4 * i + 2]};
+    P[i].restlength = (T)pos[4 * i + 3];
+    P[i].velocity = {(T)vel[4 * i + 0], (T)vel[4 * i + 1], (T)vel[4 * i + 2]};
+    last[i] = P[i].position;
+  }
+  // Integrate (:179-193)
+  for (int i = 0; i < N; ++i) {
+    Particle<T>& p = P[i];
+    if (i > 0) {
+      const V3<T> w = wind(k, p.position);
+      const V3<T> F = {k.force.x + w.x, k.force.y + w.y, k.force.z + w.z};
+      p.position = {std::fma(k.dt2, F.x, std::fma(k.dt, p.velocity.x, p.position.x)),
+                    std::fma(k.dt2, F.y, std::fma(k.dt, p.velocity.y, p.position.y)),
+                    std::fma(k.dt2, F.z, std::fma(k.dt, p.velocity.z, p.position.z))};
+    } else {
+      V3<T> src = last[0];
+      if (k.rmode == 1) src = {(T)root[0], (T)root[1], (T)root[2]};
+      p.position = transform(k, src);
+      p.velocity = sub(p.position, last[0]);
+    }
+  }
+  // SatisfyConstraints (:155-161)
+  for (int it = 0; it < k.iters; ++it) {
+    // DistanceConstraint, lane-0 body (:109-122).  Only lane 0's damping value
+    // reaches the shift loop, so the per-index curve of :100-102 never applies.
+    for (int i = 1; i < N; ++i) {
+      const V3<T> p0 = P[i - 1].position;
+      const V3<T> p1 = P[i].position;
+      const V3<T> d = sub(p1, p0);
+      const T s = (k.scale * P[i].restlength) * rsqrt_t(dot3(d, d));
+      const V3<T> nb = {std::fma(s, d.x, p0.x), std::fma(s, d.y, p0.y), std::fma(s, d.z, p0.z)};
+      P[i].position = nb;
+      P[i].velocity = sub(nb, p1);
+    }
+    for (int i = 1; i < N - 1; ++i) {
+      const V3<T> v = P[i + 1].velocity;
+      P[i].velocity = {v.x * k.damp, v.y * k.damp, v.z * k.damp};
+    }
+    // CollisionConstraint (:149-153): every sphere in order, root excluded.
+    for (int i = 1; i < N; ++i)
+      for (int s = 0; s < k.ncol; ++s) collide_sphere(k, s, P[i]);
+  }
+  // DFTL Eq. 9 and VelocityControl (:200-204), disabled in the shipped shader.
+  if (k.vmode == 1) {
+    for (int i = 0; i < N; ++i) {
+      Particle<T>& p = P[i];
+      const V3<T> a = sub(p.position, last[i]);
+      V3<T> v = {(a.x - p.velocity.x) * k.inv_dt, (a.y - p.velocity.y) * k.inv_dt,
+                 (a.z - p.velocity.z) * k.inv_dt};
+      const T dp = dot3(v, v);
+      if (dp > k.maxv2) {
+        const T f = k.maxv * rsqrt_t(dp);
+        v = {v.x * f, v.y * f, v.z * f};
+      }
+      p.velocity = v;
+    }
+  }
+  // PackParticle (:63-66)
+  for (int i = 0; i < N; ++i) {
+    pos[4 * i + 0] = (S)P[i].position.x;
+    pos[4 * i + 1] = (S)P[i].position.y;
+    pos[4 * i + 2] = (S)P[i].position.z;
+    pos[4 * i + 3] = (S)P[i].restlength;
+    vel[4 * i + 0] = (S)P[i].velocity.x;
+    vel[4 * i + 1] = (S)P[i].velocity.y;
+    vel[4 * i + 2] = (S)P[i].velocity.z;
+    vel[4 * i + 3] = S(0);
+  }
+}
+
+template <typename T, typename S>
+int step_all(const oracle_params* P, int nstrands, S* pos, S* vel, const S* roots, int nthreads) {
+  if (!P || P->npoints < 1 || P->npoints > 64 || nstrands < 0) return 1;
+  if (P->root_mode == 1 && !roots) return 1;
+  const StepConsts<T> k = make_consts<T>(P);
+  const int N = P->npoints;
+  (void)nthreads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int s = 0; s < nstrands; ++s) {
+    step_strand<T, S>(k, pos + (size_t)4 * N * s, vel + (size_t)4 * N * s,
+                      roots ? roots + (size_t)4 * s : nullptr);
+  }
+  return 0;
+}
+
+// --- dual-quaternion skinning (inc_skinning.glsl:22-31, 54-82) ------------
+
+template <typename T>
+V3<T> cross_t(V3<T> a, V3<T> b) {
+  return {std::fma(a.y, b.z, -(a.z * b.y)), std::fma(a.z, b.x, -(a.x * b.z)),
+          std::fma(a.x, b.y, -(a.y * b.x))};
+}
+
+template <typename T>
+T sign_t(T x) { return x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0)); }
+
+template <typename T>
+T dot4(const T* a, const T* b) {
+  return std::fma(a[3], b[3], std::fma(a[2], b[2], std::fma(a[1], b[1], a[0] * b[0])));
+}
+
+template <typename T>
+void skin_vertex(const float* dq, const int* joints, const float* weights, const float* rest,
+                 float* out) {
+  V3<T> v = {(T)rest[0], (T)rest[1], (T)rest[2]};
+  T w[4] = {(T)weights[0], (T)weights[1], (T)weights[2], T(0)};
+  if (w[0] <= T(1e-6f)) {  // Epsilon() (inc_constants.glsl:6)
+    out[0] = (float)v.x; out[1] = (float)v.y; out[2] = (float)v.z; out[3] = rest[3];
+    return;
+  }
+  w[3] = T(1) - (w[0] + w[1] + w[2]);
+  T Ma[4][4], Mb[4][4];
+  for (int j = 0; j < 4; ++j)
+    for (int c = 0; c < 4; ++c) {
+      Ma[j][c] = (T)dq[8 * joints[j] + c];
+      Mb[j][c] = (T)dq[8 * joints[j] + 4 + c];
+    }
+  // weights.xyz *= sign(Ma[3] * mat3x4(Ma))
+  for (int j = 0; j < 3; ++j) w[j] = w[j] * sign_t(dot4(Ma[3], Ma[j]));
+  T A[4], B[4];
+  for (int c = 0; c < 4; ++c) {
+    A[c] = std::fma(Ma[3][c], w[3], std::fma(Ma[2][c], w[2], std::fma(Ma[1][c], w[1], Ma[0][c] * w[0])));
+    B[c] = std::fma(Mb[3][c], w[3], std::fma(Mb[2][c], w[2], std::fma(Mb[1][c], w[1], Mb[0][c] * w[0])));
+  }
+  const T inv = rsqrt_t(dot4(A, A));
+  for (int c = 0; c < 4; ++c) {
+    A[c] = A[c] * inv;
+    B[c] = B[c] * inv;
+  }
+  const V3<T> a = {A[0], A[1], A[2]}, b = {B[0], B[1], B[2]};
+  // v += 2 cross(a, cross(a, v) + a.w v)
+  const V3<T> cv = cross_t(a, v);
+  const V3<T> t = {std::fma(A[3], v.x, cv.x), std::fma(A[3], v.y, cv.y), std::fma(A[3], v.z, cv.z)};
+  const V3<T> r = cross_t(a, t);
+  v = {std::fma(T(2), r.x, v.x), std::fma(T(2), r.y, v.y), std::fma(T(2), r.z, v.z)};
+  // v += 2 (a.w b - b.w a + cross(a, b))
+  const V3<T> ab = cross_t(a, b);
+  const V3<T> tr = {std::fma(A[3], b.x, std::fma(-B[3], a.x, ab.x)),
+                    std::fma(A[3], b.y, std::fma(-B[3], a.y, ab.y)),
+                    std::fma(A[3], b.z, std::fma(-B[3], a.z, ab.z))};
+  v = {std::fma(T(2), tr.x, v.x), std::fma(T(2), tr.y, v.y), std::fma(T(2), tr.z, v.z)};
+  out[0] = (float)v.x; out[1] = (float)v.y; out[2] = (float)v.z; out[3] = rest[3];
+}
+
+}  // namespace
+
+extern "C" {
+
+float oracle_rsqrt_f32(float x) { return rsqrt_t(x); }
+
+// Fills out[i] = rsqrt(in[i]) for a whole array (exhaustive accuracy checks).
+void oracle_rsqrt_f32_array(const float* in, float* out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = rsqrt_t(in[i]);
+}
+
+// One step of every strand, in place.  pos/vel: float4 [nstrands * npoints];
+// roots: float4 [nstrands] (root_mode 1) or NULL.
+int oracle_step_f32(const oracle_params* P, int nstrands, float* pos, float* vel, const float* roots) {
+  return step_all<float, float>(P, nstrands, pos, vel, roots, 1);
+}
+int oracle_step_f32_mt(const oracle_params* P, int nstrands, float* pos, float* vel,
+                       const float* roots, int nthreads) {
+  return step_all<float, float>(P, nstrands, pos, vel, roots, nthreads);
+}
+int oracle_step_f64(const oracle_params* P, int nstrands, double* pos, double* vel, const double* roots) {
+  return step_all<double, double>(P, nstrands, pos, vel, roots, 1);
+}
+
+// Wind acceleration at one point (the value CalculateForces adds to the force).
+void oracle_wind_f32(const oracle_params* P, const float* x, float* out) {
+  const StepConsts<float> k = make_consts<float>(P);
+  const V3<float> w = wind(k, {x[0], x[1], x[2]});
+  out[0] = w.x; out[1] = w.y; out[2] = w.z;
+}
+
+// Dual-quaternion blended positions.  dq: 8 floats per bone (real xyzw,
+// dual xyzw, the glm::dualquat memory order); joints: int4 per vertex;
+// weights: float4 per vertex (w ignored and derived); rest/out: float4.
+void oracle_skin_f32(int nverts, const float* dq, const int* joints, const float* weights,
+                     const float* rest, float* out) {
+  for (int v = 0; v < nverts; ++v)
+    skin_vertex<float>(dq, joints + 4 * v, weights + 4 * v, rest + 4 * v, out + 4 * v);
+}
+void oracle_skin_f64(int nverts, const float* dq, const int* joints, const float* weights,
+                     const float* rest, float* out_as_f32) {
+  for (int v = 0; v < nverts; ++v)
+    skin_vertex<double>(dq, joints + 4 * v, weights + 4 * v, rest + 4 * v, out_as_f32 + 4 * v);
+}
+
+// Root of each strand from its bound triangle: w0*A + w1*B + w2*C with
+// w1, w2 the stored barycentrics and w0 = (1 - w1) - w2, evaluated as
+// fma(w2, C, fma(w1, B, w0 * A)).  tris: int3 per triangle; bind: per strand
+// {triangle index, w1, w2} as int + 2 floats; verts/out: float4.
+void oracle_bary_roots_f32(int nstrands, const int* tris, const int* bind_tri,
+                           const float* bind_w, const float* verts, float* out) {
+  for (int s = 0; s < nstrands; ++s) {
+    const int t = bind_tri[s];
+    const float w1 = bind_w[2 * s], w2 = bind_w[2 * s + 1];
+    const float w0 = (1.0f - w1) - w2;
+    const float* A = verts + 4 * tris[3 * t + 0];
+    const float* B = verts + 4 * tris[3 * t + 1];
+    const float* C = verts + 4 * tris[3 * t + 2];
+    for (int c = 0; c < 3; ++c) out[4 * s + c] = std::fma(w2, C[c], std::fma(w1, B[c], w0 * A[c]));
+    out[4 * s + 3] = 0.0f;
+  }
+}
+
+int oracle_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

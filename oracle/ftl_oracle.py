@@ -1,0 +1,187 @@
+"""ctypes wrapper of the CPU oracle (oracle/ftl_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+the CPU-baseline leg of bench.py, as the checker.  The product (barbu_amd,
+libftl.so) never imports or calls it.
+
+Build with `make -C oracle` (also done by __graft_entry__.build()).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libftl_oracle.so")
+
+
+class OracleParams(C.Structure):
+    _fields_ = [
+        ("npoints", C.c_int),
+        ("iterations", C.c_int),
+        ("velocity_mode", C.c_int),
+        ("root_mode", C.c_int),
+        ("dt", C.c_float),
+        ("inv_dt", C.c_float),
+        ("length_scale", C.c_float),
+        ("damping", C.c_float),
+        ("max_velocity", C.c_float),
+        ("force", C.c_float * 3),
+        ("ncolliders", C.c_int),
+        ("spheres", (C.c_float * 4) * 4),
+        ("model", C.c_float * 12),
+        ("model_is_identity", C.c_int),
+        ("wind_grid", C.c_int),
+        ("wind_frequency", C.c_float),
+        ("wind_offset", C.c_float * 3),
+        ("wind_table", C.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.oracle_rsqrt_f32.restype = C.c_float
+        L.oracle_rsqrt_f32.argtypes = [C.c_float]
+        L.oracle_rsqrt_f32_array.argtypes = [vp, vp, C.c_int64]
+        for n in ("oracle_step_f32", "oracle_step_f64"):
+            getattr(L, n).restype = C.c_int
+            getattr(L, n).argtypes = [C.POINTER(OracleParams), C.c_int, vp, vp, vp]
+        L.oracle_step_f32_mt.restype = C.c_int
+        L.oracle_step_f32_mt.argtypes = [C.POINTER(OracleParams), C.c_int, vp, vp, vp, C.c_int]
+        L.oracle_wind_f32.argtypes = [C.POINTER(OracleParams), vp, vp]
+        L.oracle_skin_f32.argtypes = [C.c_int, vp, vp, vp, vp, vp]
+        L.oracle_skin_f64.argtypes = [C.c_int, vp, vp, vp, vp, vp]
+        L.oracle_bary_roots_f32.argtypes = [C.c_int, vp, vp, vp, vp, vp]
+        L.oracle_max_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def f32(x):
+    return np.float32(x)
+
+
+def make_params(npoints, dt, *, iterations=8, velocity_mode=0, root_mode=0, length_scale=1.45,
+                damping=0.80, max_velocity=100.0, gravity=(0.0, -9.81, 0.0), force_scale=20.0,
+                spheres=((0.0, 0.0, 0.0, 1.0),), model=None, wind_table=None, wind_frequency=2.0,
+                wind_offset=(0.0, 0.0, 0.0)):
+    """Oracle parameters.  Derived values are formed in float32 exactly as a GLSL
+    host would hand them over: force = force_scale * gravity, inv_dt = 1 / dt.
+    model: 4x4 row-major affine matrix or None (identity)."""
+    P = OracleParams()
+    P.npoints = int(npoints)
+    P.iterations = int(iterations)
+    P.velocity_mode = int(velocity_mode)
+    P.root_mode = int(root_mode)
+    P.dt = f32(dt)
+    P.inv_dt = f32(1.0) / f32(dt)
+    P.length_scale = f32(length_scale)
+    P.damping = f32(damping)
+    P.max_velocity = f32(max_velocity)
+    for c in range(3):
+        P.force[c] = f32(force_scale) * f32(gravity[c])
+    spheres = [] if spheres is None else list(spheres)
+    P.ncolliders = len(spheres)
+    for k, s in enumerate(spheres):
+        for c in range(4):
+            P.spheres[k][c] = f32(s[c])
+    if model is None:
+        m = np.eye(4, dtype=np.float32)
+        P.model_is_identity = 1
+    else:
+        m = np.asarray(model, np.float32).reshape(4, 4)
+        P.model_is_identity = int(np.array_equal(m, np.eye(4, dtype=np.float32)))
+    for r in range(3):
+        for c in range(4):
+            P.model[4 * r + c] = m[r, c]
+    P._keep = None
+    if wind_table is not None:
+        wt = np.ascontiguousarray(wind_table, np.float32)
+        P._keep = wt
+        P.wind_grid = int(round(wt.shape[0] ** (1.0 / 3.0)))
+        P.wind_frequency = f32(wind_frequency)
+        for c in range(3):
+            P.wind_offset[c] = f32(wind_offset[c])
+        P.wind_table = wt.ctypes.data
+    return P
+
+
+def step_f32(P, pos, vel, roots=None, nthreads=1):
+    """One step in place on float32 arrays [S, N, 4]."""
+    assert pos.dtype == np.float32 and vel.dtype == np.float32 and pos.flags.c_contiguous
+    S = pos.shape[0]
+    if nthreads > 1:
+        r = lib().oracle_step_f32_mt(C.byref(P), S, _p(pos), _p(vel), _p(roots), int(nthreads))
+    else:
+        r = lib().oracle_step_f32(C.byref(P), S, _p(pos), _p(vel), _p(roots))
+    if r != 0:
+        raise ValueError("oracle_step_f32 rejected its arguments")
+
+
+def step_f64(P, pos, vel, roots=None):
+    """One step in place on float64 arrays [S, N, 4] (same formulas in double)."""
+    assert pos.dtype == np.float64 and vel.dtype == np.float64 and pos.flags.c_contiguous
+    r = lib().oracle_step_f64(C.byref(P), pos.shape[0], _p(pos), _p(vel), _p(roots))
+    if r != 0:
+        raise ValueError("oracle_step_f64 rejected its arguments")
+
+
+def rsqrt_f32(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    lib().oracle_rsqrt_f32_array(_p(x), _p(out), x.size)
+    return out
+
+
+def wind_f32(P, x):
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+    out = np.empty_like(x)
+    for i in range(x.shape[0]):
+        lib().oracle_wind_f32(C.byref(P), _p(x[i]), _p(out[i]))
+    return out
+
+
+def skin_f32(dq, joints, weights, rest):
+    dq = np.ascontiguousarray(dq, np.float32)
+    joints = np.ascontiguousarray(joints, np.int32)
+    weights = np.ascontiguousarray(weights, np.float32)
+    rest = np.ascontiguousarray(rest, np.float32)
+    out = np.empty_like(rest)
+    lib().oracle_skin_f32(rest.shape[0], _p(dq), _p(joints), _p(weights), _p(rest), _p(out))
+    return out
+
+
+def skin_f64(dq, joints, weights, rest):
+    dq = np.ascontiguousarray(dq, np.float32)
+    joints = np.ascontiguousarray(joints, np.int32)
+    weights = np.ascontiguousarray(weights, np.float32)
+    rest = np.ascontiguousarray(rest, np.float32)
+    out = np.empty_like(rest)
+    lib().oracle_skin_f64(rest.shape[0], _p(dq), _p(joints), _p(weights), _p(rest), _p(out))
+    return out
+
+
+def bary_roots_f32(tris, tri, bary, verts):
+    tris = np.ascontiguousarray(tris, np.int32)
+    tri = np.ascontiguousarray(tri, np.int32)
+    bary = np.ascontiguousarray(bary, np.float32)
+    verts = np.ascontiguousarray(verts, np.float32)
+    out = np.empty((tri.shape[0], 4), np.float32)
+    lib().oracle_bary_roots_f32(tri.shape[0], _p(tris), _p(tri), _p(bary), _p(verts), _p(out))
+    return out
+
+
+def max_threads():
+    return lib().oracle_max_threads()

@@ -1,0 +1,342 @@
+"""GPU parity: the HIP step (libftl.so, through the C ABI) against the FP32
+oracle (oracle/ftl_oracle.cpp) on the same inputs, step by step.
+
+Tolerance (the stated bar): per step, |GPU - oracle| <= 1e-5 x strand length on
+every position and velocity component.  The kernels perform the same IEEE
+operations as the FP32 oracle, so results are also expected to be bit-identical,
+which is asserted separately.
+"""
+import numpy as np
+import pytest
+
+import barbu_amd as ba
+from oracle import ftl_oracle as orc
+from tests.helpers import assert_bitexact, max_abs, oracle_params_for
+
+pytestmark = pytest.mark.gpu
+
+DT = np.float32(1.0 / 60.0)
+TOL_REL = 1e-5  # of strand length
+
+
+def strand_length(pos):
+    return float(np.sum(pos[0, 1:, 3])) if pos.shape[1] > 1 else 1.0
+
+
+def c1_params():
+    # 4,096 strands x 16 points, gravity (0,-9.81,0), damping 0.9, 1 sphere r=1 at 0
+    return ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0)
+
+
+def run_compare(ctx, pos, vel, params, spheres, steps, *, lanes=0, roots=None, root_mode=0, models=None,
+                times=None, bitexact=True, label=""):
+    """Steps a groom on the GPU and the oracle in lockstep; checks every step."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    g = ba.Groom(ctx, pos, vel, rest_roots=roots)
+    g.set_params(params)
+    g.set_colliders(np.asarray(spheres, np.float32).reshape(-1, 4))
+    g.set_lanes(lanes)
+    if root_mode:
+        g.set_root_mode(root_mode)
+    wind = ba.wind_table(params) if params.wind_amplitude != 0.0 else None
+    P, V = pos.copy(), vel.copy()
+    L = strand_length(pos)
+    worst = 0.0
+    for k in range(steps):
+        model = models(k) if models else None
+        t = times(k) if times else 0.0
+        if model is not None:
+            g.set_root_transform(model)
+        g.set_time(t)
+        g.step(DT)
+        op = oracle_params_for(params, pos.shape[1], DT, spheres=[tuple(s) for s in np.reshape(spheres, (-1, 4))],
+                               model=model, root_mode=root_mode, time=t, wind=wind)
+        orc.step_f32(op, P, V, roots)
+        gp, gv = g.download()
+        d = max(max_abs(gp, P), max_abs(gv, V))
+        worst = max(worst, d)
+        assert d <= TOL_REL * L, "%s step %d: max |gpu - oracle| = %g > %g" % (label, k, d, TOL_REL * L)
+        if bitexact:
+            assert_bitexact(gp, P, "%s step %d positions" % (label, k))
+            assert_bitexact(gv, V, "%s step %d velocities" % (label, k))
+    g.close()
+    return P, V, worst
+
+
+def test_c1_workload_100_steps(ctx):
+    """4,096 x 16, C1 parameters, 100 steps; FP32 error vs FP64 reported."""
+    pos, vel, _ = ba.sphere_groom(4096, 16, radius=1.0, length=0.2, jitter=0.05, seed=1, coherent=False)
+    params = c1_params()
+    P, V, worst = run_compare(ctx, pos, vel, params, [(0, 0, 0, 1)], 100, label="c1")
+    # FP32 (oracle == GPU) vs FP64 after 1 and after 100 steps
+    P32, V32 = pos.copy(), vel.copy()
+    P64, V64 = pos.astype(np.float64), vel.astype(np.float64)
+    op = oracle_params_for(params, 16, DT, spheres=[(0, 0, 0, 1)])
+    errs = {}
+    for k in range(100):
+        orc.step_f32(op, P32, V32)
+        orc.step_f64(op, P64, V64)
+        if k in (0, 99):
+            errs[k + 1] = (max_abs(P32, P64), max_abs(V32, V64))
+    print("\nFP32 vs FP64 after 1 step: pos %.3g vel %.3g; after 100 steps: pos %.3g vel %.3g (strand length 0.2)"
+          % (errs[1][0], errs[1][1], errs[100][0], errs[100][1]))
+    assert np.array_equal(P32.view(np.uint32), P.view(np.uint32))
+    assert errs[1][0] < 1e-5
+
+
+def test_reference_preset(ctx):
+    """Shipped shader constants (sigma 1.45, delta 0.80, force 20 g, 4 points, one
+    sphere) on a groom built like Hair::init_simulation."""
+    rng = np.random.default_rng(7)
+    n = rng.normal(size=(448, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    roots = (0.25 * n).astype(np.float32)
+    pos, vel = ba.reference_groom(roots, n.astype(np.float32), npoints=4, maxlength=0.5, seed=3)
+    run_compare(ctx, pos, vel, ba.reference_params(), [(0.0, 0.0, 0.0, 0.25)], 12, label="reference")
+
+
+CLASSES = [(4, 1), (8, 1), (16, 1), (16, 2), (16, 4), (32, 1), (32, 2), (32, 4), (64, 2), (64, 4), (64, 1)]
+GUARDED = [2, 3, 5, 7, 12, 20, 33, 48]
+
+
+@pytest.mark.parametrize("npoints,lanes", CLASSES)
+def test_launch_classes(ctx, npoints, lanes):
+    pos, vel, _ = ba.sphere_groom(1000, npoints, radius=1.0, length=0.3, seed=npoints * 10 + lanes)
+    params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.45)
+    spheres = [(0, 0, 0, 1), (0.3, 1.1, 0.0, 0.35)]
+    run_compare(ctx, pos, vel, params, spheres, 3, lanes=lanes, label="N=%d G=%d" % (npoints, lanes))
+
+
+@pytest.mark.parametrize("npoints", GUARDED)
+def test_guarded_point_counts(ctx, npoints):
+    pos, vel, _ = ba.sphere_groom(333, npoints, radius=1.0, length=0.25, seed=npoints)
+    params = ba.make_params(force_scale=1.0, damping=0.85, length_scale=1.2)
+    run_compare(ctx, pos, vel, params, [(0, 0, 0, 1)], 3, label="N=%d" % npoints)
+
+
+def test_dftl_mode_with_clamp(ctx):
+    pos, vel, _ = ba.sphere_groom(700, 16, radius=1.0, length=0.2, seed=11)
+    vel[:, 1:, :3] = np.random.default_rng(1).normal(scale=3.0, size=vel[:, 1:, :3].shape)
+    params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0,
+                            velocity_mode=ba.FTL_VELOCITY_DFTL, max_velocity=2.0)
+    run_compare(ctx, pos, vel, params, [(0, 0, 0, 1), (0.5, 0.8, 0.0, 0.4)], 5, label="dftl")
+
+
+def test_wind_and_rotating_emitter(ctx):
+    """C2 features at small size: value-noise wind, emitter rotating about y at
+    1 rad/s (rest roots), two colliders (one orbiting)."""
+    pos, vel, roots = ba.sphere_groom(2000, 32, radius=1.0, length=0.2, seed=5)
+    params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0, wind_amplitude=2.0,
+                            wind_frequency=2.0, wind_scroll=(0.5, 0.1, -0.3), wind_seed=99)
+
+    def model(k):
+        a = float(k + 1) * float(DT)
+        c, s = np.cos(a), np.sin(a)
+        return np.array([[c, 0, s, 0], [0, 1, 0, 0], [-s, 0, c, 0], [0, 0, 0, 1]], np.float32)
+
+    spheres = [(0, 0, 0, 1), (1.2, 0.0, 0.0, 0.3)]
+    run_compare(ctx, pos, vel, params, spheres, 6, roots=roots, root_mode=ba.FTL_ROOT_REST, models=model,
+                times=lambda k: (k + 1) * float(DT), label="wind")
+
+
+@pytest.mark.parametrize("ncol", [0, 3, 4])
+def test_collider_counts_and_iterations(ctx, ncol):
+    pos, vel, _ = ba.sphere_groom(500, 8, radius=1.0, length=0.4, seed=ncol)
+    spheres = [(0, 0, 0, 1), (0.2, 1.0, 0.1, 0.3), (-0.4, 0.9, 0.0, 0.25), (0.0, -1.2, 0.0, 0.5)][:ncol]
+    for iters in (1, 3):
+        params = ba.make_params(force_scale=20.0, damping=0.8, length_scale=1.45, iterations=iters)
+        run_compare(ctx, pos, vel, params, spheres if ncol else np.zeros((0, 4)), 2,
+                    label="ncol=%d K=%d" % (ncol, iters))
+
+
+def test_model_transform_root(ctx):
+    """FTL_ROOT_MODEL with a non-identity uModel applied to the current roots."""
+    pos, vel, _ = ba.sphere_groom(300, 16, seed=21)
+    m = np.array([[1, 0, 0, 0.01], [0, 1, 0, -0.02], [0, 0, 1, 0.005], [0, 0, 0, 1]], np.float32)
+    run_compare(ctx, pos, vel, c1_params(), [(0, 0, 0, 1)], 3, models=lambda k: m, label="model")
+
+
+def test_partial_tiles_and_single_strand(ctx):
+    for S in (1, 63, 65, 129):
+        pos, vel, _ = ba.sphere_groom(S, 32, seed=S)
+        for lanes in (1, 2, 4):
+            run_compare(ctx, pos, vel, c1_params(), [(0, 0, 0, 1)], 2, lanes=lanes, label="S=%d G=%d" % (S, lanes))
+
+
+def test_nonfinite_flag(ctx):
+    pos, vel, _ = ba.sphere_groom(200, 16, seed=4)
+    g = ba.Groom(ctx, pos, vel)
+    g.step(DT)
+    assert g.status() is False
+    vel[17, 5, 0] = np.nan
+    g.upload(pos, vel)
+    g.step(DT)
+    assert g.status() is True
+    p, v = g.download()
+    assert np.isnan(p[17, -1]).any()
+    assert np.isfinite(p[16]).all() and np.isfinite(p[18]).all()
+    assert g.status() is False  # cleared by the read
+
+
+def _mixed_grooms(ctx, seed=0):
+    rng = np.random.default_rng(seed)
+    grooms, states = [], []
+    for i, (S, N) in enumerate([(1000, 8), (130, 16), (777, 32), (64, 64), (2049, 16), (5, 32), (900, 64)]):
+        pos, vel, roots = ba.sphere_groom(S, N, center=tuple(rng.normal(size=3) * 2), length=float(rng.uniform(0.05, 0.4)),
+                                          seed=100 + i)
+        g = ba.Groom(ctx, pos, vel, rest_roots=roots)
+        params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0)
+        g.set_params(params)
+        c = roots[0, :3] - np.asarray([0, 0, 0])
+        spheres = np.array([[*(pos[0, 0, :3] * 0 + np.mean(roots[:, :3], axis=0)), 1.0],
+                            [*(np.mean(roots[:, :3], axis=0) + [0.0, 1.1, 0.0]), 0.3]], np.float32)[: 1 + i % 2]
+        g.set_colliders(spheres)
+        g.set_root_mode(ba.FTL_ROOT_REST)
+        grooms.append(g)
+        states.append(dict(pos=pos.copy(), vel=vel.copy(), roots=roots, params=params, spheres=spheres, N=N))
+        del c
+    return grooms, states
+
+
+def _oracle_frame(states, dts, models):
+    for s, dt, m in zip(states, dts, models):
+        op = oracle_params_for(s["params"], s["N"], dt, spheres=[tuple(x) for x in s["spheres"]], model=m,
+                               root_mode=ba.FTL_ROOT_REST)
+        orc.step_f32(op, s["pos"], s["vel"], s["roots"])
+
+
+def _rigid(k, i):
+    a = 0.3 * (k + 1) + i
+    c, s = np.cos(a), np.sin(a)
+    return np.array([[c, -s, 0, 0.01 * k], [s, c, 0, 0.0], [0, 0, 1, -0.02 * i], [0, 0, 0, 1]], np.float32)
+
+
+def test_batched_step_matches_oracle(ctx):
+    grooms, states = _mixed_grooms(ctx)
+    for k in range(3):
+        models = [_rigid(k, i) for i in range(len(grooms))]
+        for g, m in zip(grooms, models):
+            g.set_root_transform(m)
+        dts = [np.float32(1 / 60), np.float32(1 / 90)] * 4
+        dts = dts[: len(grooms)]
+        ba.batch_step(ctx, grooms, dts)
+        _oracle_frame(states, dts, models)
+        for i, (g, s) in enumerate(zip(grooms, states)):
+            p, v = g.download()
+            assert_bitexact(p, s["pos"], "batch groom %d step %d pos" % (i, k))
+            assert_bitexact(v, s["vel"], "batch groom %d step %d vel" % (i, k))
+
+
+def test_graph_replays_match_oracle(ctx):
+    """Captured frames replayed several times, with eager work and
+    synchronisation between replays; every replay checked."""
+    grooms, states = _mixed_grooms(ctx, seed=3)
+    b = ba.Batch(ctx, grooms)
+    ntiles, grid = b.info()
+    assert ntiles > 0 and grid > 0
+    for k in range(4):
+        models = [_rigid(k, i) for i in range(len(grooms))]
+        for g, m in zip(grooms, models):
+            g.set_root_transform(m)
+        dts = [np.float32(1 / 60)] * len(grooms)
+        b.launch(dts)
+        _oracle_frame(states, dts, models)
+        # eager work between replays
+        grooms[0].download()
+        for i, (g, s) in enumerate(zip(grooms, states)):
+            p, v = g.download()
+            assert_bitexact(p, s["pos"], "graph groom %d replay %d pos" % (i, k))
+            assert_bitexact(v, s["vel"], "graph groom %d replay %d vel" % (i, k))
+    b.close()
+
+
+def test_alone_equals_batched(ctx):
+    """Each groom's result is the same stepped alone and in the mix."""
+    grooms_a, _ = _mixed_grooms(ctx, seed=9)
+    grooms_b, _ = _mixed_grooms(ctx, seed=9)
+    for k in range(2):
+        for i, (ga, gb) in enumerate(zip(grooms_a, grooms_b)):
+            m = _rigid(k, i)
+            ga.set_root_transform(m)
+            gb.set_root_transform(m)
+            ga.step(DT)
+        ba.batch_step(ctx, grooms_b, [DT] * len(grooms_b))
+    for ga, gb in zip(grooms_a, grooms_b):
+        pa, va = ga.download()
+        pb, vb = gb.download()
+        assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+        assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
+
+
+def test_skinned_roots(ctx):
+    sc = ba.skinned_scene(3000, 16, ntris=2000, nbones=16, seed=2)
+    skin = ba.Skin(ctx, sc["rest"], sc["joints"], sc["weights"], sc["tris"], sc["nbones"])
+    g = ba.Groom(ctx, sc["pos"], sc["vel"])
+    g.bind_skin(skin, sc["tri"], sc["bary"])
+    params = c1_params()
+    g.set_params(params)
+    g.set_colliders(np.array([[0, 0, 0, 1.0]], np.float32))
+    P, V = sc["pos"].copy(), sc["vel"].copy()
+    for k in range(3):
+        dq = ba.bones(sc, (k + 1) * float(DT) * 10)
+        skin.set_bones(dq)
+        skin.update()
+        g.step(DT)
+        verts = orc.skin_f32(dq, sc["joints"], sc["weights"], sc["rest"])
+        assert_bitexact(skin.download(), verts, "skinned vertices step %d" % k)
+        roots = orc.bary_roots_f32(sc["tris"], sc["tri"], sc["bary"], verts)
+        op = oracle_params_for(params, 16, DT, spheres=[(0, 0, 0, 1.0)], root_mode=ba.FTL_ROOT_REST)
+        orc.step_f32(op, P, V, roots)
+        gp, gv = g.download()
+        assert_bitexact(gp, P, "skinned step %d pos" % k)
+        assert_bitexact(gv, V, "skinned step %d vel" % k)
+    g.close()
+    skin.close()
+
+
+def test_c2_full_size_properties(ctx):
+    """262,144 x 32 (the single-groom workload): a few steps checked against the
+    oracle on a strand sample, plus size-independent invariants on all strands:
+    finite state, roots on the rotating emitter, segment lengths where no
+    collider touches, and non-penetration of the last collider."""
+    S, N = 262144, 32
+    pos, vel, roots = ba.sphere_groom(S, N, radius=1.0, length=0.2, seed=2024)
+    params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0, wind_amplitude=2.0,
+                            wind_frequency=2.0, wind_scroll=(0.4, 0.0, 0.2), wind_seed=7)
+    g = ba.Groom(ctx, pos, vel, rest_roots=roots)
+    g.set_params(params)
+    g.set_root_mode(ba.FTL_ROOT_REST)
+    wind = ba.wind_table(params)
+    sample = np.arange(0, S, 97)
+    P, V, R = pos[sample].copy(), vel[sample].copy(), roots[sample].copy()
+    for k in range(4):
+        t = (k + 1) * float(DT)
+        c, s = np.cos(t), np.sin(t)
+        m = np.array([[c, 0, s, 0], [0, 1, 0, 0], [-s, 0, c, 0], [0, 0, 0, 1]], np.float32)
+        sph = np.array([[0, 0, 0, 1], [1.2 * np.cos(t), 0, 1.2 * np.sin(t), 0.3]], np.float32)
+        g.set_root_transform(m)
+        g.set_colliders(sph)
+        g.set_time(t)
+        g.step(DT)
+        op = oracle_params_for(params, N, DT, spheres=[tuple(x) for x in sph], model=m, root_mode=ba.FTL_ROOT_REST,
+                               time=t, wind=wind)
+        orc.step_f32(op, P, V, R)
+    gp, gv = g.download()
+    assert_bitexact(gp[sample], P, "c2 sample pos")
+    assert_bitexact(gv[sample], V, "c2 sample vel")
+    assert np.isfinite(gp).all() and np.isfinite(gv).all()
+    # roots follow the emitter
+    r = roots[:, :3] @ m[:3, :3].T
+    assert np.abs(gp[:, 0, :3] - r).max() < 1e-5
+    # non-penetration of the last collider
+    d1 = np.linalg.norm(gp[:, 1:, :3] - sph[1, :3], axis=-1)
+    assert d1.min() >= 0.3 * (1 - 1e-5)
+    # inextensibility for strands no collider reaches
+    far0 = np.linalg.norm(gp[:, :, :3], axis=-1).min(axis=1) > 1.0 + 1e-3
+    far1 = np.linalg.norm(gp[:, :, :3] - sph[1, :3], axis=-1).min(axis=1) > 0.3 + 1e-3
+    free = far0 & far1
+    seg = np.linalg.norm(np.diff(gp[free, :, :3], axis=1), axis=-1)
+    assert free.sum() > 0
+    assert np.abs(seg / gp[free, 1:, 3] - 1.0).max() < 1e-4
+    g.close()
