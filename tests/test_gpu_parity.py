@@ -333,8 +333,8 @@ def test_c2_full_size_properties(ctx):
     d1 = np.linalg.norm(gp[:, 1:, :3] - sph[1, :3], axis=-1)
     assert d1.min() >= 0.3 * (1 - 1e-5)
     # inextensibility for strands no collider reaches
-    far0 = np.linalg.norm(gp[:, :, :3], axis=-1).min(axis=1) > 1.0 + 1e-3
-    far1 = np.linalg.norm(gp[:, :, :3] - sph[1, :3], axis=-1).min(axis=1) > 0.3 + 1e-3
+    far0 = np.linalg.norm(gp[:, 1:, :3], axis=-1).min(axis=1) > 1.0 + 1e-3
+    far1 = np.linalg.norm(gp[:, 1:, :3] - sph[1, :3], axis=-1).min(axis=1) > 0.3 + 1e-3
     free = far0 & far1
     seg = np.linalg.norm(np.diff(gp[free, :, :3], axis=1), axis=-1)
     assert free.sum() > 0
