@@ -151,6 +151,9 @@ SIGNATURES = {
     "ftl_batch_info": (st, [vp, ip, ip]),
     "ftl_wind_table": (st, [C.POINTER(Params), vp, ip]),
     "ftl_autotune": (st, [vp, C.c_char_p, C.c_int]),
+    "ftl_diag_copy": (st, [vp, vp, vp, C.c_size_t, vp]),
+    "ftl_diag_alloc": (st, [vp, C.c_size_t, pp]),
+    "ftl_diag_free": (st, [vp, vp, C.c_size_t]),
     "ftl_lanes_for": (C.c_int, [vp, C.c_int]),
     "ftl_scn_sphere_groom": (st, [C.POINTER(ScnSphere), vp, vp, vp]),
     "ftl_scn_reference_groom": (st, [C.c_int, vp, vp, C.c_int, C.c_float, C.c_uint64, vp, vp]),

@@ -2,9 +2,9 @@
 
     python -m barbu_amd.build           # or barbu_amd.build.build()
 
-Every launch class of the plain kernel is its own translation unit
-(ftl_plain.hip with -DFTL_CLASS_ID=k) and the persistent batched kernel is
-built once per register budget, so the objects compile in parallel.
+Every launch class is its own translation unit (ftl_plain.hip with
+-DFTL_CLASS_ID=k: the plain kernel and the persistent batched kernel of that
+class), so the objects compile in parallel.
 """
 import concurrent.futures
 import os
@@ -18,7 +18,6 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libftl.so")
 NCLASSES = 15
-BATCH_MMAX = (16, 32)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [
@@ -36,8 +35,6 @@ def _units():
     units = []
     for k in range(NCLASSES):
         units.append(("ftl_plain.hip", ["-DFTL_CLASS_ID=%d" % k], "plain_%d.o" % k))
-    for m in BATCH_MMAX:
-        units.append(("ftl_batch.hip", ["-DFTL_BATCH_MMAX=%d" % m], "batch_m%d.o" % m))
     for src in ("ftl_misc.hip", "ftl_api.cpp", "ftl_batch_api.cpp", "ftl_autotune.cpp",
                 "ftl_scenario.cpp"):
         units.append((src, [], os.path.splitext(src)[0] + ".o"))

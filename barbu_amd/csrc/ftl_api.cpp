@@ -665,3 +665,33 @@ extern "C" ftl_status ftl_groom_bind_skin(ftl_groom* g, ftl_skin* s, const int* 
   g->root_mode = FTL_ROOT_SKINNED;
   return FTL_OK;
 }
+
+// ---------------------------------------------------------------------------
+// diagnostics
+
+extern "C" ftl_status ftl_diag_copy(ftl_ctx* ctx, const void* src, void* dst, size_t bytes, void* stream) {
+  if (!ctx || !src || !dst || bytes % 16) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "bad copy arguments");
+  hipError_t e;
+  if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
+  if ((e = ftl_launch_copy(src, dst, bytes / 16, ctx->ncu, (hipStream_t)stream)) != hipSuccess)
+    return ftl_hip_fail(e, "copy launch");
+  return FTL_OK;
+}
+
+extern "C" ftl_status ftl_diag_alloc(ftl_ctx* ctx, size_t bytes, void** out) {
+  if (!ctx || !out) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  hipError_t e;
+  if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
+  ftl_status st = ftl_dev_alloc(ctx, out, bytes, "diagnostic buffer");
+  if (st != FTL_OK) return st;
+  if ((e = hipMemset(*out, 0, bytes)) != hipSuccess) return ftl_hip_fail(e, "hipMemset");
+  return FTL_OK;
+}
+
+extern "C" ftl_status ftl_diag_free(ftl_ctx* ctx, void* p, size_t bytes) {
+  if (!ctx) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  ftl_dev_free(ctx, p, bytes);
+  return FTL_OK;
+}

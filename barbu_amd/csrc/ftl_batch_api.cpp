@@ -1,15 +1,17 @@
-// Batched FTL steps: many grooms, uneven strand and point counts, one
-// persistent launch per frame.  There is no counterpart in the reference,
-// which dispatches one workgroup per strand of its single groom
-// (src/fx/hair.cc:112).
+// Batched FTL steps: many grooms, uneven strand and point counts, stepped by
+// one graph launch per frame.  There is no counterpart in the reference, which
+// dispatches one workgroup per strand of its single groom (src/fx/hair.cc:112).
 //
 // A batch is planned once: each groom is cut into 64-lane tiles of its launch
-// class, and the tiles are ordered by decreasing cost (longest first) so the
-// persistent grid drains the heavy tiles before the light ones.  A frame is
-// [upload the per-groom descriptors from pinned memory -> reset the tile
-// counter -> persistent kernel], captured once into a HIP graph.  Two graphs
-// with their own pinned descriptor buffers alternate, so the host prepares
-// frame k+1 while frame k runs.
+// class (points per lane x lanes per strand), and the tiles are grouped by
+// class.  Every class has its own persistent kernel, compiled with the
+// register budget of that class (one kernel for all classes would carry the
+// register count of the heaviest one).  A frame is [upload the per-groom
+// descriptors from pinned memory -> reset the tile counters -> one persistent
+// kernel per class present, as parallel branches], captured once into a HIP
+// graph: one graph launch steps every groom.  Two graphs with their own pinned
+// descriptor buffers alternate, so the host prepares frame k+1 while frame k
+// runs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,25 +23,38 @@
 #include "ftl_internal.h"
 #include "ftl_launch.h"
 
+// One persistent launch: the tiles [first, first + count) of one class.
+struct ClassRun {
+  int cls, first, count, grid;
+};
+
 struct ftl_batch {
   ftl_ctx* ctx = nullptr;
   std::vector<ftl_groom*> grooms;
-  int ntiles = 0, grid = 0, mmax = 16;
+  std::vector<ClassRun> runs;
+  int ntiles = 0, grid = 0;
   FtlTile* d_tiles = nullptr;
   FtlStep* d_steps[2] = {nullptr, nullptr};
   FtlStep* h_steps[2] = {nullptr, nullptr};
   unsigned* d_counter[2] = {nullptr, nullptr};
   hipStream_t cap = nullptr;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> side_ev;
+  hipEvent_t fork = nullptr;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   hipEvent_t done[2] = {nullptr, nullptr};
   int cur = 0;
 };
 
-// Plans the tiles of a set of grooms.  Returns the largest points-per-lane.
-static ftl_status plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::vector<FtlTile>& tiles, int* mmax) {
+static const int kCounterBytes = 256;  // up to 64 class counters
+
+// Plans the tiles of a set of grooms, grouped by launch class (heaviest class
+// first; within a class, grooms with more iterations first).
+static ftl_status plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::vector<FtlTile>& tiles,
+                             std::vector<ClassRun>& runs) {
   tiles.clear();
-  int mm = 0;
+  runs.clear();
   for (int i = 0; i < n; ++i) {
     ftl_groom* g = grooms[i];
     if (!g) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom %d is NULL", i);
@@ -49,29 +64,63 @@ static ftl_status plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std:
     const int cls = ftl_class_for(ctx, g->npoints, g->lanes, 32);
     if (cls < 0) return ftl_fail(FTL_ERR_UNSUPPORTED, "no batched launch class for npoints=%d", g->npoints);
     const FtlClassInfo& k = kFtlClasses[cls];
-    mm = std::max(mm, k.m);
     const int spt = FTL_WAVE / k.g;
     const int cost = k.m * (g->params.iterations + k.g - 1);
     for (int s0 = 0; s0 < g->nstrands; s0 += spt) tiles.push_back(FtlTile{i, s0, cls, cost});
   }
-  std::stable_sort(tiles.begin(), tiles.end(), [](const FtlTile& a, const FtlTile& b) { return a.cost > b.cost; });
-  *mmax = mm <= 16 ? 16 : 32;
+  std::vector<int> cls_cost(FTL_NCLASSES, -1);
+  for (const FtlTile& t : tiles) cls_cost[t.cls] = std::max(cls_cost[t.cls], t.cost);
+  std::stable_sort(tiles.begin(), tiles.end(), [&](const FtlTile& a, const FtlTile& b) {
+    if (a.cls != b.cls) {
+      if (cls_cost[a.cls] != cls_cost[b.cls]) return cls_cost[a.cls] > cls_cost[b.cls];
+      return a.cls < b.cls;
+    }
+    return a.cost > b.cost;
+  });
+  for (int i = 0; i < (int)tiles.size();) {
+    int j = i;
+    while (j < (int)tiles.size() && tiles[j].cls == tiles[i].cls) ++j;
+    int per_cu = ftl_persist_blocks_per_cu(tiles[i].cls);
+    if (per_cu <= 0) per_cu = 4;
+    const long cap = (long)ctx->ncu * per_cu;
+    runs.push_back(ClassRun{tiles[i].cls, i, j - i, (int)std::max(1L, std::min<long>(cap, j - i))});
+    i = j;
+  }
+  if (runs.size() > 64) return ftl_fail(FTL_ERR_UNSUPPORTED, "too many launch classes");
   return FTL_OK;
 }
 
-static int batch_grid(ftl_ctx* ctx, int mmax, int ntiles) {
-  int per_cu = ftl_batch_blocks_per_cu(mmax);
-  if (per_cu <= 0) per_cu = 4;
-  const long g = (long)ctx->ncu * per_cu;
-  return (int)std::max(1L, std::min<long>(g, ntiles));
+// Enqueues the counter reset and one persistent kernel per class on s.  With
+// side streams (graph capture) the classes after the first are forked onto
+// them and joined back, so they become parallel graph branches.
+static hipError_t enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_steps, const FtlTile* d_tiles,
+                               unsigned* d_counter, hipStream_t s, const std::vector<hipStream_t>* side,
+                               const std::vector<hipEvent_t>* side_ev, hipEvent_t fork) {
+  hipError_t e = ftl_reset_counters(d_counter, (int)runs.size(), s);
+  if (e != hipSuccess) return e;
+  const bool par = side && runs.size() > 1;
+  if (par && (e = hipEventRecord(fork, s)) != hipSuccess) return e;
+  for (size_t r = 0; r < runs.size(); ++r) {
+    hipStream_t rs = (par && r > 0) ? (*side)[r - 1] : s;
+    if (par && r > 0 && (e = hipStreamWaitEvent(rs, fork, 0)) != hipSuccess) return e;
+    const ClassRun& cr = runs[r];
+    if ((e = ftl_launch_persist(cr.cls, d_steps, d_tiles + cr.first, cr.count, d_counter + r, cr.grid, rs)) !=
+        hipSuccess)
+      return e;
+    if (par && r > 0) {
+      if ((e = hipEventRecord((*side_ev)[r - 1], rs)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(s, (*side_ev)[r - 1], 0)) != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
 }
 
 extern "C" ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int n, const float* dt, void* stream) {
   if (!ctx || (!grooms && n > 0) || (!dt && n > 0) || n < 0) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "bad arguments");
   if (n == 0) return FTL_OK;
   std::vector<FtlTile> tiles;
-  int mmax = 16;
-  ftl_status st = plan_tiles(ctx, grooms, n, tiles, &mmax);
+  std::vector<ClassRun> runs;
+  ftl_status st = plan_tiles(ctx, grooms, n, tiles, runs);
   if (st != FTL_OK) return st;
   std::vector<FtlStep> steps(n);
   for (int i = 0; i < n; ++i)
@@ -96,15 +145,14 @@ extern "C" ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int
     ctx->scratch_tiles_bytes = tb;
   }
   if (!ctx->scratch_counter &&
-      (st = ftl_dev_alloc(ctx, (void**)&ctx->scratch_counter, 256, "batch counter")) != FTL_OK)
+      (st = ftl_dev_alloc(ctx, (void**)&ctx->scratch_counter, kCounterBytes, "batch counters")) != FTL_OK)
     return st;
   if ((e = hipMemcpy(ctx->scratch_steps, steps.data(), sb, hipMemcpyHostToDevice)) != hipSuccess)
     return ftl_hip_fail(e, "hipMemcpy batch descriptors");
   if ((e = hipMemcpy(ctx->scratch_tiles, tiles.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
     return ftl_hip_fail(e, "hipMemcpy batch tiles");
-  const int grid = batch_grid(ctx, mmax, (int)tiles.size());
-  if ((e = ftl_launch_batch(mmax, (const FtlStep*)ctx->scratch_steps, (const FtlTile*)ctx->scratch_tiles,
-                            (int)tiles.size(), ctx->scratch_counter, grid, (hipStream_t)stream)) != hipSuccess)
+  if ((e = enqueue_runs(runs, (const FtlStep*)ctx->scratch_steps, (const FtlTile*)ctx->scratch_tiles,
+                        ctx->scratch_counter, (hipStream_t)stream, nullptr, nullptr, nullptr)) != hipSuccess)
     return ftl_hip_fail(e, "batched step launch");
   return FTL_OK;
 }
@@ -119,9 +167,12 @@ static void batch_release(ftl_batch* b) {
     if (b->done[k]) (void)hipEventDestroy(b->done[k]);
     if (b->h_steps[k]) (void)hipHostFree(b->h_steps[k]);
     ftl_dev_free(ctx, b->d_steps[k], b->grooms.size() * sizeof(FtlStep));
-    ftl_dev_free(ctx, b->d_counter[k], 256);
+    ftl_dev_free(ctx, b->d_counter[k], kCounterBytes);
   }
   ftl_dev_free(ctx, b->d_tiles, (size_t)b->ntiles * sizeof(FtlTile));
+  for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
+  for (hipEvent_t ev : b->side_ev) (void)hipEventDestroy(ev);
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->cap) (void)hipStreamDestroy(b->cap);
 }
 
@@ -129,8 +180,8 @@ extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, i
   if (!ctx || !grooms || !out || n <= 0) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "bad arguments");
   *out = nullptr;
   std::vector<FtlTile> tiles;
-  int mmax = 16;
-  ftl_status st = plan_tiles(ctx, grooms, n, tiles, &mmax);
+  std::vector<ClassRun> runs;
+  ftl_status st = plan_tiles(ctx, grooms, n, tiles, runs);
   if (st != FTL_OK) return st;
   hipError_t e;
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
@@ -138,8 +189,8 @@ extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, i
   b->ctx = ctx;
   b->grooms.assign(grooms, grooms + n);
   b->ntiles = (int)tiles.size();
-  b->mmax = mmax;
-  b->grid = batch_grid(ctx, mmax, b->ntiles);
+  b->runs = runs;
+  for (const ClassRun& r : runs) b->grid += r.grid;
   const size_t sb = (size_t)n * sizeof(FtlStep), tb = tiles.size() * sizeof(FtlTile);
   auto bail = [&](ftl_status s) {
     batch_release(b);
@@ -151,9 +202,22 @@ extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, i
     return bail(ftl_hip_fail(e, "hipMemcpy batch tiles"));
   if ((e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking)) != hipSuccess)
     return bail(ftl_hip_fail(e, "hipStreamCreate"));
+  if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess)
+    return bail(ftl_hip_fail(e, "hipEventCreate"));
+  for (size_t r = 1; r < runs.size(); ++r) {
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev = nullptr;
+    if ((e = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking)) != hipSuccess)
+      return bail(ftl_hip_fail(e, "hipStreamCreate"));
+    b->side.push_back(s2);
+    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+      return bail(ftl_hip_fail(e, "hipEventCreate"));
+    b->side_ev.push_back(ev);
+  }
   for (int k = 0; k < 2; ++k) {
     if ((st = ftl_dev_alloc(ctx, (void**)&b->d_steps[k], sb, "batch descriptors")) != FTL_OK) return bail(st);
-    if ((st = ftl_dev_alloc(ctx, (void**)&b->d_counter[k], 256, "batch counter")) != FTL_OK) return bail(st);
+    if ((st = ftl_dev_alloc(ctx, (void**)&b->d_counter[k], kCounterBytes, "batch counters")) != FTL_OK)
+      return bail(st);
     if ((e = hipHostMalloc((void**)&b->h_steps[k], sb, hipHostMallocDefault)) != hipSuccess)
       return bail(ftl_hip_fail(e, "hipHostMalloc"));
     std::memset(b->h_steps[k], 0, sb);
@@ -162,10 +226,12 @@ extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, i
     if ((e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return bail(ftl_hip_fail(e, "hipStreamBeginCapture"));
     hipError_t e1 = hipMemcpyAsync(b->d_steps[k], b->h_steps[k], sb, hipMemcpyHostToDevice, b->cap);
-    hipError_t e2 = ftl_launch_batch(b->mmax, b->d_steps[k], b->d_tiles, b->ntiles, b->d_counter[k], b->grid, b->cap);
+    hipError_t e2 = e1 == hipSuccess ? enqueue_runs(b->runs, b->d_steps[k], b->d_tiles, b->d_counter[k], b->cap,
+                                                    &b->side, &b->side_ev, b->fork)
+                                     : hipSuccess;
     hipError_t e3 = hipStreamEndCapture(b->cap, &b->graph[k]);
     if (e1 != hipSuccess) return bail(ftl_hip_fail(e1, "capture: descriptor upload"));
-    if (e2 != hipSuccess) return bail(ftl_hip_fail(e2, "capture: batched kernel"));
+    if (e2 != hipSuccess) return bail(ftl_hip_fail(e2, "capture: persistent kernels"));
     if (e3 != hipSuccess) return bail(ftl_hip_fail(e3, "hipStreamEndCapture"));
     if ((e = hipGraphInstantiate(&b->exec[k], b->graph[k], nullptr, nullptr, 0)) != hipSuccess)
       return bail(ftl_hip_fail(e, "hipGraphInstantiate"));

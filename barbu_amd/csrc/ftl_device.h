@@ -49,6 +49,11 @@ struct FtlTile {
 namespace ftl_dev {
 
 __device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+// c ? a : b per component (a select on float3 objects would select their
+// addresses and keep arrays of them out of registers).
+__device__ __forceinline__ float3 sel3(bool c, float3 a, float3 b) {
+  return f3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z);
+}
 __device__ __forceinline__ float3 sub3(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ __forceinline__ float3 mul3(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
 // fma(s, d, b) per component

@@ -6,8 +6,9 @@
 
 using namespace ftl_dev;
 
-__global__ void ftl_reset_counter_kernel(unsigned* counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+__global__ void ftl_reset_counters_kernel(unsigned* counters, int n) {
+  const int i = threadIdx.x;
+  if (blockIdx.x == 0 && i < n) counters[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -70,18 +71,33 @@ __global__ __launch_bounds__(256) void ftl_skin_kernel(int nverts, const float4*
   out[i] = make_float4(v.x, v.y, v.z, r.w);
 }
 
+// Streaming copy of float4s: the bandwidth reference for the step's traffic.
+__global__ __launch_bounds__(256) void ftl_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                       size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Host dispatch.
 
-#define FTL_DECL(id, M, G, GU) hipError_t ftl_launch_plain_##id(const FtlStep& A, hipStream_t stream);
+#define FTL_DECL(id, M, G, GU)                                                                    \
+  hipError_t ftl_launch_plain_##id(const FtlStep& A, hipStream_t stream);                           \
+  hipError_t ftl_launch_persist_##id(const FtlStep*, const FtlTile*, int, unsigned*, int, hipStream_t); \
+  int ftl_persist_occ_##id();
 FTL_CLASSES(FTL_DECL)
 #undef FTL_DECL
-hipError_t ftl_launch_batch_m16(const FtlStep*, const FtlTile*, int, unsigned*, int, hipStream_t);
-hipError_t ftl_launch_batch_m32(const FtlStep*, const FtlTile*, int, unsigned*, int, hipStream_t);
-int ftl_batch_occ_m16();
-int ftl_batch_occ_m32();
 
-int ftl_batch_blocks_per_cu(int mmax) { return mmax <= 16 ? ftl_batch_occ_m16() : ftl_batch_occ_m32(); }
+int ftl_persist_blocks_per_cu(int cls) {
+  switch (cls) {
+#define FTL_OC(id, M, G, GU) \
+  case id:                   \
+    return ftl_persist_occ_##id();
+    FTL_CLASSES(FTL_OC)
+#undef FTL_OC
+  }
+  return 0;
+}
 
 int ftl_num_classes() { return FTL_NCLASSES; }
 int ftl_class_m(int cls) { return (cls >= 0 && cls < FTL_NCLASSES) ? kFtlClasses[cls].m : 0; }
@@ -99,14 +115,22 @@ hipError_t ftl_launch_plain(int cls, const FtlStep& A, hipStream_t stream) {
   return hipErrorInvalidValue;
 }
 
-hipError_t ftl_launch_batch(int mmax, const FtlStep* grooms, const FtlTile* tiles, int ntiles,
-                            unsigned* counter, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL(ftl_reset_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  if (ntiles <= 0) return hipSuccess;
-  return mmax <= 16 ? ftl_launch_batch_m16(grooms, tiles, ntiles, counter, grid, stream)
-                    : ftl_launch_batch_m32(grooms, tiles, ntiles, counter, grid, stream);
+hipError_t ftl_reset_counters(unsigned* counters, int n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ftl_reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters, n);
+  return hipGetLastError();
+}
+
+hipError_t ftl_launch_persist(int cls, const FtlStep* grooms, const FtlTile* tiles, int ntiles, unsigned* counter,
+                              int grid, hipStream_t stream) {
+  switch (cls) {
+#define FTL_LB(id, M, G, GU) \
+  case id:                   \
+    return ftl_launch_persist_##id(grooms, tiles, ntiles, counter, grid, stream);
+    FTL_CLASSES(FTL_LB)
+#undef FTL_LB
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t ftl_launch_skin(int nverts, const float4* dq, const int4* joints, const float4* weights,
@@ -114,5 +138,15 @@ hipError_t ftl_launch_skin(int nverts, const float4* dq, const int4* joints, con
   if (nverts <= 0) return hipSuccess;
   hipLaunchKernelGGL(ftl_skin_kernel, dim3((nverts + 255) / 256), dim3(256), 0, stream, nverts, dq, joints,
                      weights, rest, out);
+  return hipGetLastError();
+}
+
+hipError_t ftl_launch_copy(const void* src, void* dst, size_t n16, int ncu, hipStream_t stream) {
+  if (n16 == 0) return hipSuccess;
+  size_t blocks = (n16 + 255) / 256;
+  const size_t cap = (size_t)ncu * 16;
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(ftl_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (const float4*)src, (float4*)dst,
+                     n16);
   return hipGetLastError();
 }

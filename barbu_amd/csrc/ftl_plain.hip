@@ -25,10 +25,19 @@ using CL = ClassOf<FTL_CLASS_ID>;
 
 // Occupancy target: the waves per SIMD the class's register state allows
 // (3 x M VGPRs of positions + M of rest lengths per lane).
-constexpr int kWavesPerEu = CL::M <= 4 ? 8 : CL::M <= 8 ? 5 : CL::M <= 16 ? 3 : 2;
+constexpr int kWavesPerEu = CL::M <= 4 ? 6 : CL::M <= 8 ? 5 : CL::M <= 16 ? 3 : 2;
 
 __global__ __launch_bounds__(FTL_WAVE) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
-void ftl_plain_kernel(FtlStep A) {
+void ftl_plain_kernel(FtlStep arg) {
+  (void)arg;
+  // The descriptor is read in place from the kernel-argument segment (scalar
+  // loads, runtime-indexed collider array) instead of through a private copy
+  // of the by-value argument.
+#if defined(__HIP_DEVICE_COMPILE__)
+  const FtlStep& A = *(const FtlStep*)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+  const FtlStep& A = arg;
+#endif
   constexpr int C = ftl_tile_impl::Tile<CL::M, CL::G, CL::GU>::C;
   __shared__ float4 stage[FTL_WAVE * C];
   __shared__ float4 u0s[FTL_WAVE];
@@ -46,4 +55,49 @@ hipError_t FTL_PASTE(ftl_launch_plain_, FTL_CLASS_ID)(const FtlStep& A, hipStrea
   if (nblocks <= 0) return hipSuccess;
   hipLaunchKernelGGL(ftl_plain_kernel, dim3(nblocks), dim3(FTL_WAVE), 0, stream, A);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Persistent form of the same class for batched frames: a grid sized to the
+// device pulls 64-lane tiles (any groom with this class) from a work list with
+// an atomic counter until the list is drained.
+
+namespace {
+
+__global__ __launch_bounds__(FTL_WAVE) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
+void ftl_persist_kernel(const FtlStep* __restrict__ grooms, const FtlTile* __restrict__ tiles, int ntiles,
+                        unsigned* __restrict__ counter) {
+  constexpr int C = ftl_tile_impl::Tile<CL::M, CL::G, CL::GU>::C;
+  __shared__ float4 stage[FTL_WAVE * C];
+  __shared__ float4 u0s[FTL_WAVE];
+  __shared__ int s_next;
+  for (;;) {
+    if (threadIdx.x == 0) s_next = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_next);
+    __syncthreads();
+    if (t >= ntiles) break;  // every wave reaches this once the list is drained
+    const FtlTile td = tiles[t];
+    const int gi = __builtin_amdgcn_readfirstlane(td.groom);
+    const int s0 = __builtin_amdgcn_readfirstlane(td.strand0);
+    ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU>(grooms[gi], s0, stage, u0s);
+  }
+}
+
+}  // namespace
+
+hipError_t FTL_PASTE(ftl_launch_persist_, FTL_CLASS_ID)(const FtlStep* grooms, const FtlTile* tiles, int ntiles,
+                                                        unsigned* counter, int grid, hipStream_t stream) {
+  if (ntiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ftl_persist_kernel, dim3(grid), dim3(FTL_WAVE), 0, stream, grooms, tiles, ntiles, counter);
+  return hipGetLastError();
+}
+
+int FTL_PASTE(ftl_persist_occ_, FTL_CLASS_ID)() {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ftl_persist_kernel, FTL_WAVE, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return blocks;
 }

@@ -99,18 +99,19 @@ __device__ __forceinline__ void sweep(const FtlStep& A, float3 (&p)[M], const fl
   constexpr int J0 = (G == 1) ? 1 : 0;  // with G == 1 slot 0 is the root
 #pragma unroll
   for (int j = J0; j < M; ++j) {
-    if (GUARD && j >= nl) break;
+    // GUARD: slots j >= nl hold copies of the last point; they are swept like
+    // the others (finite, never stored) so the loop keeps a fixed shape.
     const float3 q = p[j];
     const float3 d = sub3(q, lead);
     const float s = (sg * l[j]) * rsqrt_c(dot3(d, d));
     float3 pn = fma3(s, d, lead);
-    if (G > 1 && j == 0) pn = (g == 0) ? q : pn;  // lane 0's slot 0 is the root
+    if (G > 1 && j == 0) pn = sel3(g == 0, q, pn);  // lane 0's slot 0 is the root
     p[j] = pn;
     lead = pn;
     if (FINAL) {
       const float3 c = sub3(pn, q);
       if (G > 1 && j == 0) cfirst = c;
-      if (j >= 1) {
+      if (j >= 1 && (!GUARD || j < nl)) {
         // Point j-1 is final: u = delta * c_j (root keeps its own velocity).
         float3 pm = p[j - 1];
         float3 u = mul3(c, dl);
@@ -140,6 +141,70 @@ __device__ __forceinline__ void sweep(const FtlStep& A, float3 (&p)[M], const fl
   }
 }
 
+// Lane-predicated collisions of one point: lanes with `on` false are left
+// untouched; `reflect` also reflects u (last iteration).
+__device__ __forceinline__ void collide_pred(const FtlStep& A, float3& p, float3& u, bool on, bool reflect) {
+  const int nc = A.ncol;
+#pragma unroll 1
+  for (int k = 0; k < nc; ++k) {
+    const float3 c = f3(A.sph[k][0], A.sph[k][1], A.sph[k][2]);
+    const float3 w = sub3(p, c);
+    const float dp = dot3(w, w);
+    if (on && dp < A.r2[k]) {
+      const float is = rsqrt_c(dp);
+      const float3 n = mul3(w, is);
+      p = fma3(A.sph[k][3], n, c);
+      if (reflect) {
+        const float d2 = 2.0f * dot3(n, u);
+        u = fma3(-d2, n, u);
+      }
+    }
+  }
+}
+
+// One time step of the lane pipeline (G > 1): every lane runs the same code;
+// `active` lanes perform iteration t - g, `fin` lanes are in their last
+// iteration and also form velocities and store their points.  State changes
+// are selects, so no value is live across the two sides of a divergent branch.
+template <int M, int G>
+__device__ __forceinline__ void sweep_pipe(const FtlStep& A, float3 (&p)[M], const float (&l)[M], float3 lead,
+                                           int g, bool active, bool fin, bool valid, float4* __restrict__ pp,
+                                           float4* __restrict__ vp, float3 u0, float3& last_pre, float3& cfirst,
+                                           float3& clast) {
+  float sg = A.sigma;
+  asm volatile("" : "+v"(sg));
+  const float dl = A.delta;
+  const bool st = fin && valid;
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    const float3 q = p[j];
+    const float3 d = sub3(q, lead);
+    const float s = (sg * l[j]) * rsqrt_c(dot3(d, d));
+    float3 pn = fma3(s, d, lead);
+    const bool upd = active && (j != 0 || g != 0);  // lane 0's slot 0 is the root
+    pn = sel3(upd, pn, q);
+    p[j] = pn;
+    lead = pn;
+    const float3 c = sub3(pn, q);
+    if (j == 0) cfirst = sel3(fin, c, cfirst);
+    if (j >= 1) {
+      const bool root = (j - 1 == 0) && (g == 0);
+      float3 pm = p[j - 1];
+      float3 u = mul3(c, dl);
+      if (j - 1 == 0) u = sel3(g == 0, u0, u);
+      collide_pred(A, pm, u, active && !root, fin);
+      p[j - 1] = pm;
+      if (st) store_point(A, pp, vp, j - 1, pm, l[j - 1], u);
+    }
+    if (j == M - 1) {
+      last_pre = sel3(active, pn, last_pre);
+      clast = sel3(fin, c, clast);
+      float3 dummy = f3(0.f, 0.f, 0.f);
+      collide_pred(A, p[j], dummy, active && !fin, false);  // deferred when final
+    }
+  }
+}
+
 template <int M, int G, bool GUARD>
 __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* __restrict__ stage,
                                          float4* __restrict__ u0s) {
@@ -162,41 +227,48 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
   const float3 F0 = f3(A.force[0], A.force[1], A.force[2]);
 #pragma unroll
   for (int c = 0; c < M / C; ++c) {
-    if (GUARD && c * C >= nl) break;
-    float4 xs[C], vs[C];
+    // Groups of H points: 2H loads in flight, then the points are integrated
+    // one after another (the scheduling barrier keeps the register peak of the
+    // wind gathers to one point).
+    constexpr int H = C < 4 ? C : 4;
 #pragma unroll
-    for (int k = 0; k < C; ++k) {
-      const int q = k * FTL_WAVE + lane;
-      const int r = q / C;  // owner lane
-      const int s = min(strand0 + r / G, slast);
-      int cp = (r % G) * M + c * C + (q % C);
-      if (GUARD) cp = min(cp, N - 1);
-      const size_t idx = (size_t)s * N + cp;
-      xs[k] = A.pos[idx];
-      vs[k] = A.vel[idx];
-    }
+    for (int k0 = 0; k0 < C; k0 += H) {
+      float4 xs[H], vs[H];
 #pragma unroll
-    for (int k = 0; k < C; ++k) {
-      const int q = k * FTL_WAVE + lane;
-      const int r = q / C;
-      const int j = q % C;
-      const int cp = (r % G) * M + c * C + j;
-      const float4 x = xs[k], v = vs[k];
-      float3 F = F0;
-      if (FTL_WIND_ON && A.wind_grid) {
-        const float3 w = wind_at(A, f3(x.x, x.y, x.z));
-        F = f3(F0.x + w.x, F0.y + w.y, F0.z + w.z);
+      for (int h = 0; h < H; ++h) {
+        const int q = (k0 + h) * FTL_WAVE + lane;
+        const int r = q / C;  // owner lane
+        const int s = min(strand0 + r / G, slast);
+        int cp = (r % G) * M + c * C + (q % C);
+        if (GUARD) cp = min(cp, N - 1);
+        const size_t idx = (size_t)s * N + cp;
+        xs[h] = A.pos[idx];
+        vs[h] = A.vel[idx];
       }
-      float3 pp = f3(__builtin_fmaf(dt2, F.x, __builtin_fmaf(dt, v.x, x.x)),
-                     __builtin_fmaf(dt2, F.y, __builtin_fmaf(dt, v.y, x.y)),
-                     __builtin_fmaf(dt2, F.z, __builtin_fmaf(dt, v.z, x.z)));
-      if (cp == 0) {
-        const float3 src = root_source(A, min(strand0 + r / G, slast), x);
-        pp = A.model_ident ? src : transform(A, src);
-        const float3 u = sub3(pp, f3(x.x, x.y, x.z));
-        u0s[r] = make_float4(u.x, u.y, u.z, 0.0f);
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        const int q = (k0 + h) * FTL_WAVE + lane;
+        const int r = q / C;
+        const int j = q % C;
+        const int cp = (r % G) * M + c * C + j;
+        const float4 x = xs[h], v = vs[h];
+        float3 F = F0;
+        if (FTL_WIND_ON && A.wind_grid) {
+          const float3 w = wind_at(A, f3(x.x, x.y, x.z));
+          F = f3(F0.x + w.x, F0.y + w.y, F0.z + w.z);
+        }
+        float3 pp = f3(__builtin_fmaf(dt2, F.x, __builtin_fmaf(dt, v.x, x.x)),
+                       __builtin_fmaf(dt2, F.y, __builtin_fmaf(dt, v.y, x.y)),
+                       __builtin_fmaf(dt2, F.z, __builtin_fmaf(dt, v.z, x.z)));
+        if (cp == 0) {
+          const float3 src = root_source(A, min(strand0 + r / G, slast), x);
+          pp = A.model_ident ? src : transform(A, src);
+          const float3 u = sub3(pp, f3(x.x, x.y, x.z));
+          u0s[r] = make_float4(u.x, u.y, u.z, 0.0f);
+        }
+        stage[r * C + (j ^ (r & (C - 1)))] = make_float4(pp.x, pp.y, pp.z, x.w);
+        __builtin_amdgcn_sched_barrier(0);
       }
-      stage[r * C + (j ^ (r & (C - 1)))] = make_float4(pp.x, pp.y, pp.z, x.w);
     }
     __syncthreads();
 #pragma unroll
@@ -228,14 +300,10 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
     for (int t = 0; t < K + G - 1; ++t) {
       const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
                            __shfl_up(last_pre.z, 1, G));
-      const float3 lead = (g == 0) ? p[0] : li;
+      const float3 lead = sel3(g == 0, p[0], li);
       const int it = t - g;
-      if (it >= 0 && it < K) {
-        if (it < K - 1)
-          sweep<M, G, GUARD, false>(A, p, l, lead, g, nl, valid, pp, vp, u0, last_pre, cfirst, clast);
-        else
-          sweep<M, G, GUARD, true>(A, p, l, lead, g, nl, valid, pp, vp, u0, last_pre, cfirst, clast);
-      }
+      sweep_pipe<M, G>(A, p, l, lead, g, it >= 0 && it < K, it == K - 1, valid, pp, vp, u0, last_pre, cfirst,
+                       clast);
     }
   }
 

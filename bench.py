@@ -79,8 +79,32 @@ def bench_single(ba, hipm, steps, warmup, S=262144, N=32):
     assert np.isfinite(p).all() and np.isfinite(v).all(), "non-finite state after the benchmark"
     lanes = ctx.lanes_for(N)
     g.close()
+    copy_gbs = copy_reference(ba, hipm, ctx, S * N * 32)
     ctx.close()
-    return dict(S=S, N=N, wall_s=wall, gpu_ms=gpu_ms, lanes=lanes)
+    return dict(S=S, N=N, wall_s=wall, gpu_ms=gpu_ms, lanes=lanes, copy_gbs=copy_gbs)
+
+
+def copy_reference(ba, hipm, ctx, nbytes, reps=20):
+    """GB/s of a plain streaming copy of nbytes (the step's read + write
+    volume is 2 x nbytes): the practical bandwidth ceiling on this device."""
+    import ctypes as C
+    L = ba.lib()
+    a, b = C.c_void_p(), C.c_void_p()
+    ba.check(L.ftl_diag_alloc(ctx.handle, nbytes, C.byref(a)), "ftl_diag_alloc")
+    ba.check(L.ftl_diag_alloc(ctx.handle, nbytes, C.byref(b)), "ftl_diag_alloc")
+    stream = hipm.Stream()
+    e0, e1 = hipm.Event(), hipm.Event()
+    for _ in range(3):
+        ba.check(L.ftl_diag_copy(ctx.handle, a, b, nbytes, stream.h), "ftl_diag_copy")
+    e0.record(stream.h)
+    for _ in range(reps):
+        ba.check(L.ftl_diag_copy(ctx.handle, a, b, nbytes, stream.h), "ftl_diag_copy")
+    e1.record(stream.h)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    L.ftl_diag_free(ctx.handle, a, nbytes)
+    L.ftl_diag_free(ctx.handle, b, nbytes)
+    return 2.0 * nbytes / (ms / 1000.0) / 1e9
 
 
 def cpu_baseline(ba, seconds_target=12.0):
@@ -111,6 +135,116 @@ def cpu_baseline(ba, seconds_target=12.0):
                        "OpenMP over strands" % (S, N, k, el))
 
 
+def c3_grooms(seed=3, ngrooms=64):
+    """Composition of the batched workload: 64 grooms, strand counts
+    log-uniform in [1024, 262144], points per strand from {8,16,32,64},
+    lengths U(0.05, 0.4), a random rigid motion each (rotation about a random
+    axis at 0.5-2 rad/s plus a swaying translation), 1-3 sphere colliders that
+    move with the groom (a head sphere r=1 at its centre, 0-2 more)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(ngrooms):
+        S = int(round(np.exp(rng.uniform(np.log(1024), np.log(262144)))))
+        N = int(rng.choice([8, 16, 32, 64]))
+        L = float(rng.uniform(0.05, 0.4))
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        w = float(rng.uniform(0.5, 2.0))
+        sway = rng.uniform(0.0, 0.2, size=3)
+        ncol = int(rng.integers(1, 4))
+        extra = []
+        for _ in range(ncol - 1):
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            extra.append((*(d * rng.uniform(1.0, 1.5)), rng.uniform(0.2, 0.5)))
+        center = np.array([(i % 8) * 3.0, 0.0, (i // 8) * 3.0])
+        out.append(dict(S=S, N=N, L=L, axis=axis, w=w, sway=sway, spheres=[(0, 0, 0, 1.0)] + extra,
+                        center=center, seed=1000 + i))
+    return out
+
+
+def rigid(c, t):
+    """Row-major 4x4 world transform of groom c at time t."""
+    a = c["w"] * t
+    k = c["axis"]
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    R = np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * K @ K
+    m = np.eye(4)
+    m[:3, :3] = R
+    m[:3, 3] = c["center"] + c["sway"] * np.sin(2.0 * t)
+    return m.astype(np.float32)
+
+
+def world_spheres(c, m):
+    sp = np.array(c["spheres"], np.float32)
+    out = sp.copy()
+    out[:, :3] = sp[:, :3] @ m[:3, :3].T + m[:3, 3]
+    return out
+
+
+def make_c3(ba, ctx, comp):
+    grooms = []
+    params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0, iterations=8)
+    for c in comp:
+        pos, vel, roots = ba.sphere_groom(c["S"], c["N"], radius=1.0, length=c["L"], jitter=0.05, seed=c["seed"])
+        g = ba.Groom(ctx, pos, vel, rest_roots=roots)
+        g.set_params(params)
+        g.set_root_mode(ba.FTL_ROOT_REST)
+        grooms.append(g)
+        del pos, vel, roots
+    return grooms
+
+
+def c3_frame_inputs(comp, grooms, t):
+    for c, g in zip(comp, grooms):
+        m = rigid(c, t)
+        g.set_root_transform(m)
+        g.set_colliders(world_spheres(c, m))
+
+
+def bench_batched(ba, hipm, frames, warmup, comp=None):
+    comp = comp or c3_grooms()
+    ctx = ba.Context(0)
+    grooms = make_c3(ba, ctx, comp)
+    cps = sum(c["S"] * c["N"] for c in comp)
+    stream = hipm.Stream()
+    e0, e1 = hipm.Event(), hipm.Event()
+    batch = ba.Batch(ctx, grooms)
+    ntiles, grid = batch.info()
+    dts = np.full(len(grooms), DT, np.float32)
+    res = {}
+    for mode in ("batched", "plain"):
+        for k in range(warmup):
+            c3_frame_inputs(comp, grooms, (k + 1) * float(DT))
+            if mode == "batched":
+                batch.launch(dts, stream.h)
+            else:
+                for g in grooms:
+                    g.step(DT, stream.h)
+        hipm.device_synchronize()
+        w0 = time.perf_counter()
+        e0.record(stream.h)
+        for k in range(warmup, warmup + frames):
+            c3_frame_inputs(comp, grooms, (k + 1) * float(DT))
+            if mode == "batched":
+                batch.launch(dts, stream.h)
+            else:
+                for g in grooms:
+                    g.step(DT, stream.h)
+        e1.record(stream.h)
+        hipm.device_synchronize()
+        wall = time.perf_counter() - w0
+        res[mode] = dict(wall_s=wall, gpu_ms=e0.elapsed_ms(e1))
+    for g in grooms:
+        p, v = g.download()
+        assert np.isfinite(p).all() and np.isfinite(v).all(), "non-finite state after the batched benchmark"
+    batch.close()
+    for g in grooms:
+        g.close()
+    ctx.close()
+    return dict(cps=cps, ngrooms=len(comp), ntiles=ntiles, grid=grid, **res)
+
+
 def load_traffic(workload):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -127,6 +261,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--workload", default="all", choices=["single", "batched", "all"])
+    ap.add_argument("--batched-frames", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (strands and grooms shard with no "
@@ -136,6 +272,8 @@ def main():
     import barbu_amd as ba
     from barbu_amd import _hip as hipm
 
+    if args.workload == "batched":
+        args.steps = max(args.steps, 1)
     log("single groom: 262144 x 32, %d warmup + %d timed steps" % (args.warmup, args.steps))
     r = bench_single(ba, hipm, args.steps, args.warmup)
     cps = r["S"] * r["N"]
@@ -163,8 +301,28 @@ def main():
                    "strands": r["S"], "points": r["N"], "lanes_per_strand": r["lanes"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_us": launch_s * 1e6, "bytes_per_launch": cps * BYTES_PER_CP},
+                     "kernel_us": launch_s * 1e6, "bytes_per_launch": cps * BYTES_PER_CP,
+                     "copy_reference_gbs": r["copy_gbs"]},
     }
+    if args.workload in ("batched", "all"):
+        log("batched: 64 uneven grooms, %d frames (graph) and %d frames of per-groom launches"
+            % (args.batched_frames, args.batched_frames))
+        b = bench_batched(ba, hipm, args.batched_frames, 3)
+        fb = b["batched"]
+        fp = b["plain"]
+        v_b = b["cps"] * args.batched_frames / fb["wall_s"]
+        v_p = b["cps"] * args.batched_frames / fp["wall_s"]
+        gb = b["cps"] * BYTES_PER_CP / (fb["gpu_ms"] / 1000.0 / args.batched_frames) / 1e9
+        out["batched"] = {
+            "workload": "64 grooms, strands log-uniform [1024, 262144], points {8,16,32,64}, lengths U(0.05,0.4), "
+                        "random rigid motion, 1-3 spheres; one graph launch per frame",
+            "control_points": b["cps"], "tiles": b["ntiles"], "grid": b["grid"], "frames": args.batched_frames,
+            "value": v_b, "unit": "CP updates/s", "ms_per_frame": 1000.0 * fb["wall_s"] / args.batched_frames,
+            "gpu_ms_per_frame": fb["gpu_ms"] / args.batched_frames,
+            "achieved_gbs": gb, "frac_hbm_peak": gb / HBM_PEAK_GBS,
+            "plain_value": v_p, "plain_ms_per_frame": 1000.0 * fp["wall_s"] / args.batched_frames,
+            "speedup_vs_plain": v_b / v_p,
+        }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores)")
         out["cpu_baseline"] = cpu_baseline(ba, args.cpu_seconds)

@@ -225,6 +225,15 @@ ftl_status ftl_batch_info(ftl_batch* b, int* ntiles, int* grid);
    (xyz acceleration, w = 0), entry (z*G + y)*G + x.  CPU only. */
 ftl_status ftl_wind_table(const ftl_params* p, float* out, int* grid);
 
+/* Diagnostics: copies `bytes` (a multiple of 16) from src to dst, both device
+   pointers, with a plain streaming kernel on `stream`.  The bandwidth
+   reference the step's HBM traffic is compared with.  No counterpart in the
+   reference. */
+ftl_status ftl_diag_copy(ftl_ctx* ctx, const void* src, void* dst, size_t bytes, void* stream);
+/* Diagnostics: device buffer helpers for the copy above. */
+ftl_status ftl_diag_alloc(ftl_ctx* ctx, size_t bytes, void** out);
+ftl_status ftl_diag_free(ftl_ctx* ctx, void* p, size_t bytes);
+
 /* -- launch-shape tuning ---------------------------------------------------- */
 
 /* Times the candidate lanes-per-strand for every point count on the current
