@@ -47,9 +47,19 @@ def c2_params(ba):
                           wind_amplitude=2.0, wind_frequency=2.0, wind_scroll=(0.5, 0.0, 0.25), wind_seed=2)
 
 
+TUNED = os.path.join(ROOT, "barbu_amd", "tuned_gfx950.json")
+
+
+def context(ba):
+    """A context with the lanes-per-strand table measured on MI355X."""
+    ctx = ba.Context(0)
+    ctx.autotune(TUNED)  # loads the table (no timing) when it matches the device
+    return ctx
+
+
 def bench_single(ba, hipm, steps, warmup, S=262144, N=32):
     pos, vel, roots = ba.sphere_groom(S, N, radius=1.0, length=0.2, jitter=0.05, seed=2024)
-    ctx = ba.Context(0)
+    ctx = context(ba)
     g = ba.Groom(ctx, pos, vel, rest_roots=roots)
     g.set_params(c2_params(ba))
     g.set_root_mode(ba.FTL_ROOT_REST)
@@ -187,6 +197,10 @@ def make_c3(ba, ctx, comp):
     params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0, iterations=8)
     for c in comp:
         pos, vel, roots = ba.sphere_groom(c["S"], c["N"], radius=1.0, length=c["L"], jitter=0.05, seed=c["seed"])
+        # initial state at the groom's world placement at t = 0 (roots stay in
+        # object space: the step applies the transform)
+        m0 = rigid(c, 0.0)
+        pos[..., :3] = pos[..., :3] @ m0[:3, :3].T + m0[:3, 3]
         g = ba.Groom(ctx, pos, vel, rest_roots=roots)
         g.set_params(params)
         g.set_root_mode(ba.FTL_ROOT_REST)
@@ -204,7 +218,7 @@ def c3_frame_inputs(comp, grooms, t):
 
 def bench_batched(ba, hipm, frames, warmup, comp=None):
     comp = comp or c3_grooms()
-    ctx = ba.Context(0)
+    ctx = context(ba)
     grooms = make_c3(ba, ctx, comp)
     cps = sum(c["S"] * c["N"] for c in comp)
     stream = hipm.Stream()

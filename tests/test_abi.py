@@ -1,0 +1,140 @@
+"""CPU tests of the C ABI library: it loads, exports every symbol its headers
+declare, reports errors without a GPU, and its host-side generators behave.
+No kernel is launched here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import barbu_amd as ba
+from barbu_amd import _ffi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("ftl.h", "ftl_scenario.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(ftl_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(_ffi.LIB_PATH)
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_ctypes_signatures_cover_the_headers():
+    assert declared_functions() <= set(_ffi.SIGNATURES), declared_functions() - set(_ffi.SIGNATURES)
+
+
+def test_abi_version_and_reference_params():
+    assert ba.lib().ftl_abi_version() == 1
+    p = ba.reference_params()
+    assert tuple(p.gravity) == (0.0, np.float32(-9.81), 0.0)
+    assert p.force_scale == 20.0 and p.iterations == 8
+    assert p.damping == np.float32(0.80) and p.length_scale == np.float32(1.45)
+    assert p.velocity_mode == ba.FTL_VELOCITY_SHIPPED and p.max_velocity == 100.0
+    assert p.wind_amplitude == 0.0
+
+
+def test_init_without_device_reports_an_error():
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK | os.W_OK):
+        pytest.skip("a GPU is present")
+    h = C.c_void_p()
+    st = ba.lib().ftl_init(0, C.byref(h))
+    assert st == _ffi.FTL_ERR_DEVICE
+    assert b"device" in ba.lib().ftl_last_error()
+    assert not h.value
+
+
+def test_null_arguments_are_rejected():
+    L = ba.lib()
+    assert L.ftl_groom_step(None, C.c_float(0.01), None) == _ffi.FTL_ERR_INVALID_ARGUMENT
+    assert L.ftl_groom_set_colliders(None, None, 0) == _ffi.FTL_ERR_INVALID_ARGUMENT
+    assert L.ftl_batch_launch(None, None, None) == _ffi.FTL_ERR_INVALID_ARGUMENT
+    assert b"NULL" in L.ftl_last_error() or b"null" in L.ftl_last_error().lower()
+
+
+def test_sphere_groom_generator():
+    pos, vel, roots = ba.sphere_groom(500, 16, center=(1, 2, 3), radius=0.5, length=0.3, seed=7)
+    pos2, vel2, roots2 = ba.sphere_groom(500, 16, center=(1, 2, 3), radius=0.5, length=0.3, seed=7)
+    assert np.array_equal(pos, pos2) and np.array_equal(roots, roots2)
+    assert not np.array_equal(pos, ba.sphere_groom(500, 16, center=(1, 2, 3), radius=0.5, length=0.3, seed=8)[0])
+    r = np.linalg.norm(roots[:, :3] - np.array([1, 2, 3]), axis=1)
+    np.testing.assert_allclose(r, 0.5, rtol=1e-5)
+    np.testing.assert_array_equal(pos[:, 0, :3], roots[:, :3])
+    assert np.all(pos[:, 0, 3] == 0.0)
+    np.testing.assert_allclose(pos[:, 1:, 3], 0.3 / 15, rtol=1e-6)
+    seg = np.linalg.norm(np.diff(pos[:, :, :3].astype(np.float64), axis=1), axis=-1)
+    np.testing.assert_allclose(seg, 0.3 / 15, rtol=1e-4)
+    assert np.all(vel == 0)
+    # coherent order is a permutation of generation order
+    p3, _, r3 = ba.sphere_groom(500, 16, center=(1, 2, 3), radius=0.5, length=0.3, seed=7, coherent=False)
+    a = np.lexsort(r3[:, :3].T)
+    b = np.lexsort(roots[:, :3].T)
+    np.testing.assert_array_equal(r3[a], roots[b])
+
+
+def test_reference_groom_follows_init_simulation():
+    """hair.cc:258-288: offset_i = i * (maxlength/N) * r_j, w_i = offset_i -
+    offset_{i-1}, pos = root + offset * normal, r_j in [0.9, 1.1]."""
+    rng = np.random.default_rng(0)
+    roots = rng.normal(size=(20, 3)).astype(np.float32)
+    n = rng.normal(size=(20, 3))
+    n = (n / np.linalg.norm(n, axis=1, keepdims=True)).astype(np.float32)
+    pos, vel = ba.reference_groom(roots, n, npoints=4, maxlength=0.5, seed=1)
+    assert pos.shape == (20, 4, 4) and np.all(vel == 0)
+    assert np.all(pos[:, 0, 3] == 0)
+    np.testing.assert_array_equal(pos[:, 0, :3], roots)
+    offs = np.cumsum(pos[:, :, 3], axis=1)
+    rj = offs[:, 1] / np.float32(0.5 / 4)
+    assert np.all((rj >= 0.9 - 1e-6) & (rj <= 1.1 + 1e-6))
+    np.testing.assert_allclose(pos[:, :, :3], roots[:, None, :] + offs[:, :, None] * n[:, None, :], atol=1e-6)
+
+
+def test_wind_table_deterministic_and_bounded():
+    p = ba.make_params(wind_amplitude=2.0, wind_seed=5)
+    t1, t2 = ba.wind_table(p), ba.wind_table(p)
+    assert t1.shape == (ba.FTL_WIND_GRID ** 3, 4)
+    assert np.array_equal(t1, t2)
+    assert np.abs(t1[:, :3]).max() <= 2.0 and np.all(t1[:, 3] == 0)
+    assert abs(float(t1[:, :3].mean())) < 0.1
+    assert not np.array_equal(t1, ba.wind_table(ba.make_params(wind_amplitude=2.0, wind_seed=6)))
+
+
+def test_skinned_scene_generator():
+    sc = ba.skinned_scene(1000, 8, ntris=2000, nbones=16, seed=3)
+    nv = sc["rest"].shape[0]
+    assert sc["tris"].shape[0] >= 1800
+    assert sc["joints"].min() >= 0 and sc["joints"].max() < 16
+    np.testing.assert_allclose(sc["weights"].sum(axis=1), 1.0, rtol=1e-5)
+    assert np.all(np.diff(sc["weights"], axis=1) <= 1e-7)  # largest first
+    assert sc["tris"].min() >= 0 and sc["tris"].max() < nv
+    assert np.all(sc["bary"] >= 0) and np.all(sc["bary"].sum(axis=1) <= 1 + 1e-6)
+    dq = ba.bones(sc, 0.3)
+    np.testing.assert_allclose(np.linalg.norm(dq[:, :4], axis=1), 1.0, rtol=1e-5)
+    # real and dual parts are orthogonal for a rigid motion
+    np.testing.assert_allclose((dq[:, :4] * dq[:, 4:]).sum(axis=1), 0.0, atol=1e-5)
+    # roots lie on the bound triangles
+    t = sc["tris"][sc["tri"]]
+    v = sc["rest"][:, :3]
+    w1, w2 = sc["bary"][:, 0:1], sc["bary"][:, 1:2]
+    root = (1 - w1 - w2) * v[t[:, 0]] + w1 * v[t[:, 1]] + w2 * v[t[:, 2]]
+    np.testing.assert_allclose(sc["pos"][:, 0, :3], root, atol=1e-5)
+
+
+def test_hair_mirror_uninitialised_update_is_a_noop():
+    h = ba.Hair()
+    assert not h.initialized()
+    h.update(1 / 60)  # hair.cc:90-93: logs and returns
+    h.deinit()
+    assert not h.initialized()

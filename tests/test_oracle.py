@@ -414,3 +414,17 @@ def test_bary_roots():
     bary = np.array([[0, 0], [1, 0], [0, 1], [0.25, 0.5]], np.float32)
     r = orc.bary_roots_f32(tris, np.zeros(4, np.int32), bary, verts)
     np.testing.assert_allclose(r[:, :3], [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0.25, 0.5, 0]], atol=1e-7)
+
+
+def test_zero_length_segment_is_not_finite():
+    """A point on top of its leader: normalize(0) has no direction.  GLSL gives
+    inversesqrt(0) = +inf and 0 * inf = NaN; the FP32 inversesqrt here also
+    yields NaN for 0 (the Newton step computes 0 * inf), so the strand from
+    that point on becomes NaN, as in the reference (the step reports it through
+    the groom's non-finite flag)."""
+    assert np.isnan(orc.rsqrt_f32(np.zeros(1, np.float32))[0])
+    pos = np.array([[[0, 0, 0, 0], [0, -0.1, 0, 0.1], [0, -0.1, 0, 0.1], [0, -0.3, 0, 0.1]]], np.float32)
+    P = orc.make_params(4, F(1 / 60), iterations=1, length_scale=1.0, force_scale=0.0, spheres=None)
+    p, v = step(P, pos, np.zeros_like(pos))
+    assert np.isfinite(p[0, :2]).all()
+    assert np.isnan(p[0, 2:, :3]).all()
