@@ -4,31 +4,34 @@
 // `cls` field of batched tiles.
 #pragma once
 
-#define FTL_CLASSES(X)  \
-  X(0, 4, 1, false)     \
-  X(1, 8, 1, false)     \
-  X(2, 16, 1, false)    \
-  X(3, 32, 1, false)    \
-  X(4, 8, 2, false)     \
-  X(5, 16, 2, false)    \
-  X(6, 32, 2, false)    \
-  X(7, 4, 4, false)     \
-  X(8, 8, 4, false)     \
-  X(9, 16, 4, false)    \
-  X(10, 4, 1, true)     \
-  X(11, 8, 1, true)     \
-  X(12, 16, 1, true)    \
-  X(13, 32, 1, true)    \
-  X(14, 64, 1, true)
+// X(id, points per lane, lanes per strand, guarded, waves per SIMD targeted
+// by the register allocator: the highest at which neither the plain nor the
+// persistent kernel of the class spills)
+#define FTL_CLASSES(X)     \
+  X(0, 4, 1, false, 8)     \
+  X(1, 8, 1, false, 6)     \
+  X(2, 16, 1, false, 4)    \
+  X(3, 32, 1, false, 2)    \
+  X(4, 8, 2, false, 4)     \
+  X(5, 16, 2, false, 2)    \
+  X(6, 32, 2, false, 2)    \
+  X(7, 4, 4, false, 6)     \
+  X(8, 8, 4, false, 4)     \
+  X(9, 16, 4, false, 3)    \
+  X(10, 4, 1, true, 6)     \
+  X(11, 8, 1, true, 6)     \
+  X(12, 16, 1, true, 4)    \
+  X(13, 32, 1, true, 2)    \
+  X(14, 64, 1, true, 2)
 
 #define FTL_NCLASSES 15
 
 struct FtlClassInfo {
-  int m, g, guarded;
+  int m, g, guarded, waves;
 };
 
 static constexpr FtlClassInfo kFtlClasses[FTL_NCLASSES] = {
-#define FTL_CI(id, M, G, GU) {M, G, GU ? 1 : 0},
+#define FTL_CI(id, M, G, GU, W) {M, G, GU ? 1 : 0, W},
     FTL_CLASSES(FTL_CI)
 #undef FTL_CI
 };

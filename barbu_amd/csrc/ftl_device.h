@@ -105,32 +105,40 @@ __device__ __forceinline__ float3 wind_at(const FtlStep& A, float3 x) {
   const float s1 = (f1 * f1) * __builtin_fmaf(-2.0f, f1, 3.0f);
   const float s2 = (f2 * f2) * __builtin_fmaf(-2.0f, f2, 3.0f);
   const float4* T = A.wind;
-  const float4 v000 = T[(z0 * G + y0) * G + x0], v001 = T[(z0 * G + y0) * G + x1];
-  const float4 v010 = T[(z0 * G + y1) * G + x0], v011 = T[(z0 * G + y1) * G + x1];
-  const float4 v100 = T[(z1 * G + y0) * G + x0], v101 = T[(z1 * G + y0) * G + x1];
-  const float4 v110 = T[(z1 * G + y1) * G + x0], v111 = T[(z1 * G + y1) * G + x1];
+  // Four x-edges of the cell, gathered one edge (two corners) at a time so the
+  // register peak stays small: c_yz = lerp(v[z][y][x0], v[z][y][x1], s0).
+  const int r00 = (z0 * G + y0) * G, r10 = (z0 * G + y1) * G, r01 = (z1 * G + y0) * G, r11 = (z1 * G + y1) * G;
+  float4 a0 = T[r00 + x0], a1 = T[r00 + x1];
+  const float3 c00 = f3(lerp_c(a0.x, a1.x, s0), lerp_c(a0.y, a1.y, s0), lerp_c(a0.z, a1.z, s0));
+  __builtin_amdgcn_sched_barrier(0);
+  a0 = T[r10 + x0];
+  a1 = T[r10 + x1];
+  const float3 c10 = f3(lerp_c(a0.x, a1.x, s0), lerp_c(a0.y, a1.y, s0), lerp_c(a0.z, a1.z, s0));
+  __builtin_amdgcn_sched_barrier(0);
+  a0 = T[r01 + x0];
+  a1 = T[r01 + x1];
+  const float3 c01 = f3(lerp_c(a0.x, a1.x, s0), lerp_c(a0.y, a1.y, s0), lerp_c(a0.z, a1.z, s0));
+  __builtin_amdgcn_sched_barrier(0);
+  a0 = T[r11 + x0];
+  a1 = T[r11 + x1];
+  const float3 c11 = f3(lerp_c(a0.x, a1.x, s0), lerp_c(a0.y, a1.y, s0), lerp_c(a0.z, a1.z, s0));
   float3 r;
-#define FTL_TRI(c)                                                      \
-  {                                                                     \
-    const float c00 = lerp_c(v000.c, v001.c, s0);                       \
-    const float c10 = lerp_c(v010.c, v011.c, s0);                       \
-    const float c01 = lerp_c(v100.c, v101.c, s0);                       \
-    const float c11 = lerp_c(v110.c, v111.c, s0);                       \
-    r.c = lerp_c(lerp_c(c00, c10, s1), lerp_c(c01, c11, s1), s2);       \
-  }
-  FTL_TRI(x) FTL_TRI(y) FTL_TRI(z)
-#undef FTL_TRI
+  r.x = lerp_c(lerp_c(c00.x, c10.x, s1), lerp_c(c01.x, c11.x, s1), s2);
+  r.y = lerp_c(lerp_c(c00.y, c10.y, s1), lerp_c(c01.y, c11.y, s1), s2);
+  r.z = lerp_c(lerp_c(c00.z, c10.z, s1), lerp_c(c01.z, c11.z, s1), s2);
   return r;
 }
 
-// CollideSphere(+1, c_k, r_k, p) for every collider in order
-// (cs_simulation.glsl:129-153).  REFLECT: also reflect u on each hit; only the
-// last constraint iteration's velocity reaches the output.
+// CollideSphere(+1, c_k, r_k, p) for the colliders of `cmask`, in order
+// (cs_simulation.glsl:129-153).  cmask is wave-uniform: the colliders some
+// strand of the wave can reach (the others cannot change any result).
+// REFLECT: also reflect u on each hit; only the last constraint iteration's
+// velocity reaches the output.
 template <bool REFLECT>
-__device__ __forceinline__ void collide(const FtlStep& A, float3& p, float3& u) {
-  const int nc = A.ncol;
+__device__ __forceinline__ void collide(const FtlStep& A, float3& p, float3& u, unsigned cmask) {
 #pragma unroll 1
-  for (int k = 0; k < nc; ++k) {
+  for (unsigned m = cmask; m; m &= m - 1) {
+    const int k = __builtin_ctz(m);
     const float3 c = f3(A.sph[k][0], A.sph[k][1], A.sph[k][2]);
     const float3 w = sub3(p, c);
     const float dp = dot3(w, w);
@@ -144,6 +152,30 @@ __device__ __forceinline__ void collide(const FtlStep& A, float3& p, float3& u) 
       }
     }
   }
+}
+
+// Colliders a strand can touch during one step, as a bit mask.  After every
+// sweep each point lies within the chain length L of its root p0 (segments
+// are sigma*l long); collider j moves a point it contains to its surface, by
+// at most r_j - |p0 - c_j| + D when D bounds the point's distance to p0, so
+// D grows by that much for each collider that can touch the strand.  A
+// collider farther than r_k + D (with a generous rounding margin) can never
+// contain a point of the strand in this step.
+__device__ __forceinline__ unsigned reachable_colliders(const FtlStep& A, float3 p0, float L) {
+  float D = L * 1.001f + 1e-4f;
+  const float a = sqrtf(dot3(p0, p0));
+  unsigned mask = 0u;
+  for (int k = 0; k < A.ncol; ++k) {
+    const float3 c = f3(A.sph[k][0], A.sph[k][1], A.sph[k][2]);
+    const float r = A.sph[k][3];
+    const float dc = sqrtf(dot3(sub3(p0, c), sub3(p0, c)));
+    const float tol = 1e-4f * (a + sqrtf(dot3(c, c)) + r + D + 1.0f);
+    if (!(dc - r > D + tol)) {  // NaN keeps the collider
+      mask |= 1u << k;
+      D += fmaxf(0.0f, r - dc + D) + tol;
+    }
+  }
+  return mask;
 }
 
 }  // namespace ftl_dev

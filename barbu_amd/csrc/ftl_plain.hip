@@ -12,10 +12,10 @@ namespace {
 
 template <int ID>
 struct ClassOf;
-#define FTL_CO(id, M_, G_, GU_)              \
+#define FTL_CO(id, M_, G_, GU_, W_)              \
   template <>                                \
   struct ClassOf<id> {                       \
-    static constexpr int M = M_, G = G_;     \
+    static constexpr int M = M_, G = G_, W = W_; \
     static constexpr bool GU = GU_;          \
   };
 FTL_CLASSES(FTL_CO)
@@ -23,9 +23,12 @@ FTL_CLASSES(FTL_CO)
 
 using CL = ClassOf<FTL_CLASS_ID>;
 
-// Occupancy target: the waves per SIMD the class's register state allows
-// (3 x M VGPRs of positions + M of rest lengths per lane).
-constexpr int kWavesPerEu = CL::M <= 4 ? 6 : CL::M <= 8 ? 5 : CL::M <= 16 ? 3 : 2;
+// Occupancy target of the class (waves per SIMD the register allocator aims at).
+#ifdef FTL_WPE_OVERRIDE
+constexpr int kWavesPerEu = FTL_WPE_OVERRIDE;
+#else
+constexpr int kWavesPerEu = CL::W;
+#endif
 
 __global__ __launch_bounds__(FTL_WAVE) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
 void ftl_plain_kernel(FtlStep arg) {

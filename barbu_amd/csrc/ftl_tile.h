@@ -23,8 +23,13 @@
 //     time steps.
 //   * Only the last iteration's velocity reaches the output (every iteration
 //     overwrites u_i for i >= 1 before reading it), so velocity work, the
-//     damped shift and the reflections are done in the last iteration only,
-//     and each point is written to HBM as soon as it is final.
+//     damped shift and the reflections are done in the last iteration only.
+//   * Store: the inverse of the load, through the same LDS buffer, so every
+//     global store writes whole 128-byte lines.  With G = 1, velocities leave
+//     chunk by chunk during the last sweep and positions after it.  With G > 1
+//     the lanes of a strand finish at different time steps, so velocities are
+//     written by each lane as they become final; positions leave coalesced
+//     after the pipeline drains.
 //   * In place: a tile owns its strands, so no second buffer and no copy-back.
 
 #pragma once
@@ -43,26 +48,29 @@ enum { kRootModel = 0, kRootRest = 1, kRootSkinned = 2 };
 
 template <int M, int G, bool GUARD>
 struct Tile {
-  static constexpr int C = (M < 8) ? M : 8;  // points per lane per load chunk
+  static constexpr int C = (M < 4) ? M : 4;  // points per lane per load/store chunk
   static constexpr int SPT = FTL_WAVE / G;   // strands per tile
 };
 
-// Stores one final control point through the lane's base pointers (constant
-// offsets, so the stores use immediate offsets and no per-point address).
-__device__ __forceinline__ void store_point(const FtlStep& A, float4* __restrict__ pp,
-                                            float4* __restrict__ vp, int j, float3 p, float w, float3 u) {
-  float3 v = u;
-  if (A.vmode == kVelDftl) {
-    // v = ((p - x) - u) / dt, then VelocityControl (:79-84).  x is still in
-    // memory: this point has not been written yet.
-    const float4 x = pp[j];
-    v = f3(((p.x - x.x) - u.x) * A.inv_dt, ((p.y - x.y) - u.y) * A.inv_dt,
-           ((p.z - x.z) - u.z) * A.inv_dt);
-    const float dp = dot3(v, v);
-    if (dp > A.maxv2) v = mul3(v, A.maxv * rsqrt_c(dp));
-  }
-  pp[j] = make_float4(p.x, p.y, p.z, w);
-  vp[j] = make_float4(v.x, v.y, v.z, 0.0f);
+// LDS slot of (owner lane r, point j of the chunk): XOR swizzle, conflict-free
+// for both the per-lane and the coalesced access order.
+template <int C>
+__device__ __forceinline__ int slot(int r, int j) {
+  return r * C + (j ^ (r & (C - 1)));
+}
+
+// Stored velocity of a final point: the damped, reflected correction u as
+// shipped, or DFTL Eq. 9 with VelocityControl (:79-84, :201-204).  In DFTL
+// mode x is re-read from memory, where the point's position is still the
+// step's input (positions are written after all velocities).
+__device__ __forceinline__ float3 out_velocity(const FtlStep& A, const float4* __restrict__ pp, int j, float3 p,
+                                               float3 u) {
+  if (A.vmode != kVelDftl) return u;
+  const float4 x = pp[j];
+  float3 v = f3(((p.x - x.x) - u.x) * A.inv_dt, ((p.y - x.y) - u.y) * A.inv_dt, ((p.z - x.z) - u.z) * A.inv_dt);
+  const float dp = dot3(v, v);
+  if (dp > A.maxv2) v = mul3(v, A.maxv * rsqrt_c(dp));
+  return v;
 }
 
 __device__ __forceinline__ float3 root_source(const FtlStep& A, int s, float4 x) {
@@ -83,59 +91,85 @@ __device__ __forceinline__ float3 root_source(const FtlStep& A, int s, float4 x)
   return f3(x.x, x.y, x.z);
 }
 
-// One sweep of DistanceConstraint over this lane's points, fused with the
-// collisions of the points it has passed.  FINAL: last iteration, which also
-// forms the velocities and writes every point but the lane's last.
-template <int M, int G, bool GUARD, bool FINAL>
-__device__ __forceinline__ void sweep(const FtlStep& A, float3 (&p)[M], const float (&l)[M], float3 lead,
-                                      int g, int nl, bool valid, float4* __restrict__ pp,
-                                      float4* __restrict__ vp, float3 u0,
-                                      float3& last_pre, float3& cfirst, float3& clast) {
+// Writes chunk c of the tile (C points per lane, staged in LDS by every lane)
+// to dst in the coalesced order of the load.  Called by the whole wave.
+template <int M, int G, bool GUARD>
+__device__ __forceinline__ void flush_chunk(float4* __restrict__ dst, int strand0, int nstr, int N, int c,
+                                            const float4* __restrict__ stage, int lane) {
+  constexpr int C = Tile<M, G, GUARD>::C;
+  // Recomputed here (opaque lane): keeping the load phase's addresses alive
+  // until the stores would cost registers for the whole step.
+  int ln = lane;
+  asm volatile("" : "+v"(ln));
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    const int q = k * FTL_WAVE + ln;
+    const int r = q / C;
+    const int j = q % C;
+    const int s = strand0 + r / G;
+    const int cp = (r % G) * M + c * C + j;
+    if (s < nstr && (!GUARD || cp < N)) dst[(size_t)s * N + cp] = stage[slot<C>(r, j)];
+  }
+  __syncthreads();
+}
+
+// One sweep of DistanceConstraint over this lane's points (G == 1), fused with
+// the collisions of the points it has passed.  FINAL: last iteration, which
+// also forms the velocities, stages them and flushes every completed chunk;
+// the lane's last point is left to the caller (its velocity is the raw
+// correction and its collision comes after the sweep).
+template <int M, bool GUARD, bool FINAL>
+__device__ __forceinline__ void sweep(const FtlStep& A, float3 (&p)[M], const float (&l)[M], int nl,
+                                      const float4* __restrict__ pp, float3 u0, float3& last_pre, float3& clast,
+                                      float4* __restrict__ stage, int strand0, int N, int lane, unsigned cm) {
+  constexpr int C = Tile<M, 1, GUARD>::C;
   // Opaque per sweep: keeps the compiler from hoisting sigma * l[j] out of the
   // iteration loop into M extra registers (one multiply per point instead).
   float sg = A.sigma;
   asm volatile("" : "+v"(sg));
   const float dl = A.delta;
-  constexpr int J0 = (G == 1) ? 1 : 0;  // with G == 1 slot 0 is the root
+  float3 lead = p[0];
 #pragma unroll
-  for (int j = J0; j < M; ++j) {
+  for (int j = 1; j < M; ++j) {
     // GUARD: slots j >= nl hold copies of the last point; they are swept like
     // the others (finite, never stored) so the loop keeps a fixed shape.
     const float3 q = p[j];
     const float3 d = sub3(q, lead);
     const float s = (sg * l[j]) * rsqrt_c(dot3(d, d));
-    float3 pn = fma3(s, d, lead);
-    if (G > 1 && j == 0) pn = sel3(g == 0, q, pn);  // lane 0's slot 0 is the root
+    const float3 pn = fma3(s, d, lead);
     p[j] = pn;
     lead = pn;
     if (FINAL) {
       const float3 c = sub3(pn, q);
-      if (G > 1 && j == 0) cfirst = c;
-      if (j >= 1 && (!GUARD || j < nl)) {
-        // Point j-1 is final: u = delta * c_j (root keeps its own velocity).
-        float3 pm = p[j - 1];
+      if (!GUARD || j < nl) {
+        // Point m = j-1 is final: u = delta * c_j (the root keeps its own).
+        const int m = j - 1;
+        float3 pm = p[m];
         float3 u = mul3(c, dl);
-        if (j - 1 == 0 && (G == 1 || g == 0)) {
+        if (m == 0) {
           u = u0;
         } else {
-          collide<true>(A, pm, u);
+          collide<true>(A, pm, u, cm);
         }
-        p[j - 1] = pm;
-        if (valid) store_point(A, pp, vp, j - 1, pm, l[j - 1], u);
+        p[m] = pm;
+        const float3 v = out_velocity(A, pp, m, pm, u);
+        stage[slot<C>(lane, m % C)] = make_float4(v.x, v.y, v.z, 0.0f);
+        if (m % C == C - 1) flush_chunk<M, 1, GUARD>(A.vel, strand0, A.nstrands, N, m / C, stage, lane);
       }
       if (GUARD ? (j == nl - 1) : (j == M - 1)) {
         clast = c;
         last_pre = pn;
       }
     } else {
-      if (j >= 1 && !(j - 1 == 0 && (G == 1 || g == 0))) {
+      if (j >= 2) {
         float3 dummy;
-        collide<false>(A, p[j - 1], dummy);
+        collide<false>(A, p[j - 1], dummy, cm);
       }
       if (GUARD ? (j == nl - 1) : (j == M - 1)) {
         last_pre = pn;
         float3 dummy;
-        collide<false>(A, p[j], dummy);
+        collide<false>(A, p[j], dummy, cm);
       }
     }
   }
@@ -143,10 +177,11 @@ __device__ __forceinline__ void sweep(const FtlStep& A, float3 (&p)[M], const fl
 
 // Lane-predicated collisions of one point: lanes with `on` false are left
 // untouched; `reflect` also reflects u (last iteration).
-__device__ __forceinline__ void collide_pred(const FtlStep& A, float3& p, float3& u, bool on, bool reflect) {
-  const int nc = A.ncol;
+__device__ __forceinline__ void collide_pred(const FtlStep& A, float3& p, float3& u, bool on, bool reflect,
+                                             unsigned cmask) {
 #pragma unroll 1
-  for (int k = 0; k < nc; ++k) {
+  for (unsigned mk = cmask; mk; mk &= mk - 1) {
+    const int k = __builtin_ctz(mk);
     const float3 c = f3(A.sph[k][0], A.sph[k][1], A.sph[k][2]);
     const float3 w = sub3(p, c);
     const float dp = dot3(w, w);
@@ -164,17 +199,18 @@ __device__ __forceinline__ void collide_pred(const FtlStep& A, float3& p, float3
 
 // One time step of the lane pipeline (G > 1): every lane runs the same code;
 // `active` lanes perform iteration t - g, `fin` lanes are in their last
-// iteration and also form velocities and store their points.  State changes
-// are selects, so no value is live across the two sides of a divergent branch.
+// iteration and also write their final velocities (lanes finish at different
+// time steps, so these go straight to memory; positions leave coalesced after
+// the pipeline drains).  State changes are selects, so no value is live across
+// the two sides of a divergent branch.
 template <int M, int G>
 __device__ __forceinline__ void sweep_pipe(const FtlStep& A, float3 (&p)[M], const float (&l)[M], float3 lead,
-                                           int g, bool active, bool fin, bool valid, float4* __restrict__ pp,
+                                           int g, bool active, bool fin, bool valid, const float4* __restrict__ pp,
                                            float4* __restrict__ vp, float3 u0, float3& last_pre, float3& cfirst,
-                                           float3& clast) {
+                                           float3& clast, unsigned cm) {
   float sg = A.sigma;
   asm volatile("" : "+v"(sg));
   const float dl = A.delta;
-  const bool st = fin && valid;
 #pragma unroll
   for (int j = 0; j < M; ++j) {
     const float3 q = p[j];
@@ -192,15 +228,18 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, float3 (&p)[M], con
       float3 pm = p[j - 1];
       float3 u = mul3(c, dl);
       if (j - 1 == 0) u = sel3(g == 0, u0, u);
-      collide_pred(A, pm, u, active && !root, fin);
+      collide_pred(A, pm, u, active && !root, fin, cm);
       p[j - 1] = pm;
-      if (st) store_point(A, pp, vp, j - 1, pm, l[j - 1], u);
+      if (fin && valid) {
+        const float3 v = out_velocity(A, pp, j - 1, pm, u);
+        vp[j - 1] = make_float4(v.x, v.y, v.z, 0.0f);
+      }
     }
     if (j == M - 1) {
       last_pre = sel3(active, pn, last_pre);
       clast = sel3(fin, c, clast);
       float3 dummy = f3(0.f, 0.f, 0.f);
-      collide_pred(A, p[j], dummy, active && !fin, false);  // deferred when final
+      collide_pred(A, p[j], dummy, active && !fin, false, cm);  // deferred when final
     }
   }
 }
@@ -213,16 +252,17 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
   const int g = (G == 1) ? 0 : (lane % G);
   const int N = GUARD ? A.npoints : M * G;
   const int nl = GUARD ? A.npoints : M;  // points held by this lane (G == 1 when GUARD)
+  const int nstr = A.nstrands;
   const int strand = strand0 + lane / G;
-  const bool valid = strand < A.nstrands;
-  const int slast = A.nstrands - 1;
+  const bool valid = strand < nstr;
+  const int slast = nstr - 1;
 
   float3 p[M];
   float l[M];
 
   // ---- load + integrate, through LDS ------------------------------------
   // Loads are clamped into the groom (out-of-range lanes read valid data that
-  // is never stored), so all 2C loads of a chunk are issued back to back.
+  // is never stored), so the loads of a group are issued back to back.
   const float dt = A.dt, dt2 = A.dt2;
   const float3 F0 = f3(A.force[0], A.force[1], A.force[2]);
 #pragma unroll
@@ -230,7 +270,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
     // Groups of H points: 2H loads in flight, then the points are integrated
     // one after another (the scheduling barrier keeps the register peak of the
     // wind gathers to one point).
-    constexpr int H = C < 4 ? C : 4;
+    constexpr int H = C < 2 ? C : 2;
 #pragma unroll
     for (int k0 = 0; k0 < C; k0 += H) {
       float4 xs[H], vs[H];
@@ -257,23 +297,23 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
           const float3 w = wind_at(A, f3(x.x, x.y, x.z));
           F = f3(F0.x + w.x, F0.y + w.y, F0.z + w.z);
         }
-        float3 pp = f3(__builtin_fmaf(dt2, F.x, __builtin_fmaf(dt, v.x, x.x)),
+        float3 pr = f3(__builtin_fmaf(dt2, F.x, __builtin_fmaf(dt, v.x, x.x)),
                        __builtin_fmaf(dt2, F.y, __builtin_fmaf(dt, v.y, x.y)),
                        __builtin_fmaf(dt2, F.z, __builtin_fmaf(dt, v.z, x.z)));
         if (cp == 0) {
           const float3 src = root_source(A, min(strand0 + r / G, slast), x);
-          pp = A.model_ident ? src : transform(A, src);
-          const float3 u = sub3(pp, f3(x.x, x.y, x.z));
+          pr = A.model_ident ? src : transform(A, src);
+          const float3 u = sub3(pr, f3(x.x, x.y, x.z));
           u0s[r] = make_float4(u.x, u.y, u.z, 0.0f);
         }
-        stage[r * C + (j ^ (r & (C - 1)))] = make_float4(pp.x, pp.y, pp.z, x.w);
+        stage[slot<C>(r, j)] = make_float4(pr.x, pr.y, pr.z, x.w);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < C; ++j) {
-      const float4 t = stage[lane * C + (j ^ (lane & (C - 1)))];
+      const float4 t = stage[slot<C>(lane, j)];
       p[c * C + j] = f3(t.x, t.y, t.z);
       l[c * C + j] = t.w;
     }
@@ -285,52 +325,89 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
     u0 = f3(t.x, t.y, t.z);
   }
 
+  // ---- colliders the wave can reach this step ----------------------------
+  unsigned cm;
+  {
+    float L = 0.0f;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+      if (j >= (G == 1 ? 1 : 0) && (!GUARD || j < nl)) L += A.sigma * l[j];
+    float3 root = p[0];
+    if (G > 1) {
+      // strand total and root, from the lanes of the strand
+      for (int o = 1; o < G; o <<= 1) L += __shfl_xor(L, o, G);
+      root = f3(__shfl(root.x, 0, G), __shfl(root.y, 0, G), __shfl(root.z, 0, G));
+    }
+    const unsigned mine = valid ? reachable_colliders(A, root, L) : 0u;
+    unsigned any = 0u;
+    for (int k = 0; k < A.ncol; ++k)
+      if (__ballot((mine >> k) & 1u) != 0ull) any |= 1u << k;
+    cm = __builtin_amdgcn_readfirstlane(any);
+  }
+
   // ---- constraints ------------------------------------------------------
-  const size_t base = (size_t)(valid ? strand : 0) * N + (size_t)g * M;
-  float4* __restrict__ pp = A.pos + base;
-  float4* __restrict__ vp = A.vel + base;
+  // Lane's own points, for the DFTL re-read of x (never stored through).
+  const float4* __restrict__ pp = A.pos + (size_t)(valid ? strand : 0) * N + (size_t)g * M;
   const int K = A.iterations;
   float3 last_pre = p[0];
   float3 cfirst = f3(0.f, 0.f, 0.f), clast = f3(0.f, 0.f, 0.f);
+  float3 u;
+  bool bad = false;
   if (G == 1) {
     for (int it = 0; it < K - 1; ++it)
-      sweep<M, G, GUARD, false>(A, p, l, p[0], 0, nl, valid, pp, vp, u0, last_pre, cfirst, clast);
-    sweep<M, G, GUARD, true>(A, p, l, p[0], 0, nl, valid, pp, vp, u0, last_pre, cfirst, clast);
+      sweep<M, GUARD, false>(A, p, l, nl, pp, u0, last_pre, clast, stage, strand0, N, lane, cm);
+    sweep<M, GUARD, true>(A, p, l, nl, pp, u0, last_pre, clast, stage, strand0, N, lane, cm);
+    // The last point: velocity = raw correction, collisions deferred to here.
+    u = clast;
+#pragma unroll
+    for (int j = 1; j < M; ++j) {
+      if (GUARD ? (j == nl - 1) : (j == M - 1)) {
+        float3 pm = p[j];
+        collide<true>(A, pm, u, cm);
+        p[j] = pm;
+        const float3 v = out_velocity(A, pp, j, pm, u);
+        bad = !(isfinite(pm.x) && isfinite(pm.y) && isfinite(pm.z) && isfinite(v.x) && isfinite(v.y) &&
+                isfinite(v.z));
+        stage[slot<C>(lane, j % C)] = make_float4(v.x, v.y, v.z, 0.0f);
+        flush_chunk<M, G, GUARD>(A.vel, strand0, nstr, N, j / C, stage, lane);
+      }
+    }
   } else {
+    float4* __restrict__ vp = A.vel + (size_t)(valid ? strand : 0) * N + (size_t)g * M;
     for (int t = 0; t < K + G - 1; ++t) {
       const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
                            __shfl_up(last_pre.z, 1, G));
       const float3 lead = sel3(g == 0, p[0], li);
       const int it = t - g;
       sweep_pipe<M, G>(A, p, l, lead, g, it >= 0 && it < K, it == K - 1, valid, pp, vp, u0, last_pre, cfirst,
-                       clast);
+                       clast, cm);
     }
-  }
-
-  // ---- the lane's last point: its velocity needs the next lane's correction
-  float3 u = clast;
-  if (G > 1) {
+    // The lane's last point: its velocity needs the next lane's correction.
     const float3 cn = f3(__shfl_down(cfirst.x, 1, G), __shfl_down(cfirst.y, 1, G),
                          __shfl_down(cfirst.z, 1, G));
-    if (g != G - 1) u = mul3(cn, A.delta);
+    u = sel3(g == G - 1, clast, mul3(cn, A.delta));
+    float3 pm = p[M - 1];
+    collide<true>(A, pm, u, cm);
+    p[M - 1] = pm;
+    const float3 v = out_velocity(A, pp, M - 1, pm, u);
+    if (valid) vp[M - 1] = make_float4(v.x, v.y, v.z, 0.0f);
+    bad = !(isfinite(pm.x) && isfinite(pm.y) && isfinite(pm.z) && isfinite(v.x) && isfinite(v.y) &&
+            isfinite(v.z));
   }
-  bool bad = false;
-  // (GUARD: the last point is nl - 1, selected with uniform branches)
+  // positions, after every velocity (DFTL reads the step's input positions)
 #pragma unroll
-  for (int j = 0; j < M; ++j) {
-    if (GUARD ? (j == nl - 1) : (j == M - 1)) {
-      float3 pm = p[j];
-      collide<true>(A, pm, u);
-      p[j] = pm;
-      if (valid) {
-        store_point(A, pp, vp, j, pm, l[j], u);
-        bad = !(isfinite(pm.x) && isfinite(pm.y) && isfinite(pm.z) && isfinite(u.x) &&
-                isfinite(u.y) && isfinite(u.z));
-      }
+  for (int c = 0; c < M / C; ++c) {
+    if (GUARD && c * C >= nl) continue;
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const float3 q = p[c * C + j];
+      stage[slot<C>(lane, j)] = make_float4(q.x, q.y, q.z, l[c * C + j]);
     }
+    flush_chunk<M, G, GUARD>(A.pos, strand0, nstr, N, c, stage, lane);
   }
+  bad = bad && valid;
   const unsigned long long b = __ballot(bad);
   if (b != 0ull && lane == (int)(__ffsll((long long)b) - 1)) atomicOr(A.status, 1u);
 }
 
-}  // namespace
+}  // namespace ftl_tile_impl

@@ -340,3 +340,23 @@ def test_c2_full_size_properties(ctx):
     assert free.sum() > 0
     assert np.abs(seg / gp[free, 1:, 3] - 1.0).max() < 1e-4
     g.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_collider_reach_bound_edges(ctx, lanes):
+    """Colliders placed around the edge of what a strand can reach in one step
+    (root distance ~ radius + strand length), and chains where one collider's
+    push carries points toward the next: skipping unreachable colliders must
+    not change any result."""
+    rng = np.random.default_rng(17 + lanes)
+    S, N, L = 640, 32, 0.3
+    pos, vel, _ = ba.sphere_groom(S, N, radius=1.0, length=L, seed=40 + lanes)
+    vel[:, 1:, :3] = rng.normal(scale=2.0, size=vel[:, 1:, :3].shape)
+    params = ba.make_params(force_scale=20.0, damping=0.85, length_scale=1.0)
+    base = pos[rng.integers(0, S), 0, :3].astype(np.float64)
+    d = base / np.linalg.norm(base)
+    spheres = [(0, 0, 0, 1.0),
+               (*(base + d * (0.25 + L * 0.999)), 0.25),    # just inside the chain reach
+               (*(base + d * (0.25 + L * 1.001)), 0.25),    # just outside it
+               (*(base + d * 0.9 + rng.normal(scale=0.05, size=3)), 0.45)]
+    run_compare(ctx, pos, vel, params, spheres, 3, lanes=lanes, label="reach G=%d" % lanes)
