@@ -78,6 +78,42 @@ __device__ __forceinline__ float rsqrt_c(float x) {
   return y;
 }
 
+// One FTL point update, in place (DistanceConstraint, cs_simulation.glsl:110-116):
+//   d = p - lead;  p = lead + (sl * inversesqrt(dot(d, d))) * d
+// with the operations of dot3 / rsqrt_c / fma3 above, in the same order.
+// Written out so the new position is produced in p's own registers (the
+// compiler otherwise builds it in temporaries and copies it back every point).
+// VALU instructions only; no memory access.
+__device__ __forceinline__ void ftl_point(float3& p, const float3 lead, const float sl) {
+  float d0, d1, d2, x, y, t, s;
+  asm(
+      "v_sub_f32 %[d0], %[p0], %[l0]\n\t"
+      "v_sub_f32 %[d1], %[p1], %[l1]\n\t"
+      "v_sub_f32 %[d2], %[p2], %[l2]\n\t"
+      "v_mul_f32 %[x], %[d0], %[d0]\n\t"
+      "v_fmac_f32 %[x], %[d1], %[d1]\n\t"
+      "v_fmac_f32 %[x], %[d2], %[d2]\n\t"
+      "v_lshrrev_b32 %[y], 1, %[x]\n\t"
+      "v_sub_u32 %[y], 0x5f3759df, %[y]\n\t"
+      "v_mul_f32 %[x], -0.5, %[x]\n\t"
+      "v_mul_f32 %[t], %[y], %[y]\n\t"
+      "v_fmaak_f32 %[t], %[x], %[t], 0x3fc00000\n\t"
+      "v_mul_f32 %[y], %[y], %[t]\n\t"
+      "v_mul_f32 %[t], %[y], %[y]\n\t"
+      "v_fmaak_f32 %[t], %[x], %[t], 0x3fc00000\n\t"
+      "v_mul_f32 %[y], %[y], %[t]\n\t"
+      "v_mul_f32 %[t], %[y], %[y]\n\t"
+      "v_fmaak_f32 %[t], %[x], %[t], 0x3fc00000\n\t"
+      "v_mul_f32 %[y], %[y], %[t]\n\t"
+      "v_mul_f32 %[s], %[sl], %[y]\n\t"
+      "v_fma_f32 %[p0], %[s], %[d0], %[l0]\n\t"
+      "v_fma_f32 %[p1], %[s], %[d1], %[l1]\n\t"
+      "v_fma_f32 %[p2], %[s], %[d2], %[l2]"
+      : [p0] "+v"(p.x), [p1] "+v"(p.y), [p2] "+v"(p.z), [d0] "=&v"(d0), [d1] "=&v"(d1), [d2] "=&v"(d2),
+        [x] "=&v"(x), [y] "=&v"(y), [t] "=&v"(t), [s] "=&v"(s)
+      : [l0] "v"(lead.x), [l1] "v"(lead.y), [l2] "v"(lead.z), [sl] "v"(sl));
+}
+
 // Root transform: row c = fma(m_c0, x, fma(m_c1, y, fma(m_c2, z, t_c))).
 __device__ __forceinline__ float3 transform(const FtlStep& A, float3 x) {
   const float* m = A.model;
@@ -129,23 +165,62 @@ __device__ __forceinline__ float3 wind_at(const FtlStep& A, float3 x) {
   return r;
 }
 
+// The colliders of one step, read once per tile into scalar registers and
+// made opaque there, so the per-point collision code never reloads them.
+struct Spheres {
+  float c[FTL_NCOL][3];
+  float r[FTL_NCOL];
+  float r2[FTL_NCOL];
+};
+
+__device__ __forceinline__ float pin_s(float x) {
+  int i = __builtin_amdgcn_readfirstlane(__float_as_int(x));
+  asm volatile("" : "+s"(i));
+  return __int_as_float(i);
+}
+
+__device__ __forceinline__ Spheres load_spheres(const FtlStep& A, unsigned cmask) {
+  Spheres S;
+#pragma unroll
+  for (int k = 0; k < FTL_NCOL; ++k) {
+    if (!(cmask & (1u << k))) {
+      S.c[k][0] = S.c[k][1] = S.c[k][2] = S.r[k] = S.r2[k] = 0.0f;
+      continue;
+    }
+    S.c[k][0] = pin_s(A.sph[k][0]);
+    S.c[k][1] = pin_s(A.sph[k][1]);
+    S.c[k][2] = pin_s(A.sph[k][2]);
+    S.r[k] = pin_s(A.sph[k][3]);
+    S.r2[k] = pin_s(A.r2[k]);
+  }
+  return S;
+}
+
 // CollideSphere(+1, c_k, r_k, p) for the colliders of `cmask`, in order
 // (cs_simulation.glsl:129-153).  cmask is wave-uniform: the colliders some
 // strand of the wave can reach (the others cannot change any result).
 // REFLECT: also reflect u on each hit; only the last constraint iteration's
 // velocity reaches the output.
+// Test of bit k of the wave-uniform collider mask as a scalar branch.  The
+// mask is made opaque at each test: hoisted out of the iteration loop, the
+// test becomes a per-lane boolean that costs two vector instructions a use.
+__device__ __forceinline__ bool has_collider(unsigned cmask, int k) {
+  asm volatile("" : "+s"(cmask));
+  return (cmask >> k) & 1u;
+}
+
 template <bool REFLECT>
-__device__ __forceinline__ void collide(const FtlStep& A, float3& p, float3& u, unsigned cmask) {
-#pragma unroll 1
-  for (unsigned m = cmask; m; m &= m - 1) {
-    const int k = __builtin_ctz(m);
-    const float3 c = f3(A.sph[k][0], A.sph[k][1], A.sph[k][2]);
+__device__ __forceinline__ void collide(const Spheres& S, float3& p, float3& u, unsigned cmask) {
+#pragma unroll
+  for (int k = 0; k < FTL_NCOL; ++k) {
+    if (!has_collider(cmask, k)) continue;
+    const float3 c = f3(S.c[k][0], S.c[k][1], S.c[k][2]);
     const float3 w = sub3(p, c);
     const float dp = dot3(w, w);
-    if (dp < A.r2[k]) {
+    if (dp < S.r2[k]) {
       const float is = rsqrt_c(dp);
       const float3 n = mul3(w, is);
-      p = fma3(A.sph[k][3], n, c);
+      p = fma3(S.r[k], n, c);
       if (REFLECT) {
         const float d2 = 2.0f * dot3(n, u);
         u = fma3(-d2, n, u);

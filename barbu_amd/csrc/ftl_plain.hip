@@ -41,10 +41,8 @@ void ftl_plain_kernel(FtlStep arg) {
 #else
   const FtlStep& A = arg;
 #endif
-  constexpr int C = ftl_tile_impl::Tile<CL::M, CL::G, CL::GU>::C;
-  __shared__ float4 stage[FTL_WAVE * C];
-  __shared__ float4 u0s[FTL_WAVE];
-  ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU>(A, blockIdx.x * (FTL_WAVE / CL::G), stage, u0s);
+  __shared__ ftl_tile_impl::TileLds<CL::M, CL::G, CL::GU> sh;
+  ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU>(A, blockIdx.x * (FTL_WAVE / CL::G), sh);
 }
 
 }  // namespace
@@ -70,9 +68,7 @@ namespace {
 __global__ __launch_bounds__(FTL_WAVE) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
 void ftl_persist_kernel(const FtlStep* __restrict__ grooms, const FtlTile* __restrict__ tiles, int ntiles,
                         unsigned* __restrict__ counter) {
-  constexpr int C = ftl_tile_impl::Tile<CL::M, CL::G, CL::GU>::C;
-  __shared__ float4 stage[FTL_WAVE * C];
-  __shared__ float4 u0s[FTL_WAVE];
+  __shared__ ftl_tile_impl::TileLds<CL::M, CL::G, CL::GU> sh;
   __shared__ int s_next;
   for (;;) {
     if (threadIdx.x == 0) s_next = (int)atomicAdd(counter, 1u);
@@ -83,7 +79,7 @@ void ftl_persist_kernel(const FtlStep* __restrict__ grooms, const FtlTile* __res
     const FtlTile td = tiles[t];
     const int gi = __builtin_amdgcn_readfirstlane(td.groom);
     const int s0 = __builtin_amdgcn_readfirstlane(td.strand0);
-    ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU>(grooms[gi], s0, stage, u0s);
+    ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU>(grooms[gi], s0, sh);
   }
 }
 

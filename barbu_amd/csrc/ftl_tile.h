@@ -34,7 +34,12 @@
 
 #pragma once
 
+#include <type_traits>
+
 #include "ftl_device.h"
+#ifndef FTL_NO_DFTL
+#define FTL_NO_DFTL 0
+#endif
 #ifndef FTL_WIND_ON
 #define FTL_WIND_ON 1
 #endif
@@ -59,13 +64,28 @@ __device__ __forceinline__ int slot(int r, int j) {
   return r * C + (j ^ (r & (C - 1)));
 }
 
+// Rest lengths of the lane's points.  With one lane per strand they live in
+// LDS (point-major, so a wave's reads are conflict-free), leaving the VGPRs to
+// the positions; the pipelined forms keep them in registers.
+template <int M>
+struct LenRegs {
+  float v[M];
+  __device__ __forceinline__ float get(int j) const { return v[j]; }
+  __device__ __forceinline__ void set(int j, float x) { v[j] = x; }
+};
+struct LenLds {
+  float* base;  // lane's column: element j at base[j * FTL_WAVE]
+  __device__ __forceinline__ float get(int j) const { return base[j * FTL_WAVE]; }
+  __device__ __forceinline__ void set(int j, float x) { base[j * FTL_WAVE] = x; }
+};
+
 // Stored velocity of a final point: the damped, reflected correction u as
 // shipped, or DFTL Eq. 9 with VelocityControl (:79-84, :201-204).  In DFTL
 // mode x is re-read from memory, where the point's position is still the
 // step's input (positions are written after all velocities).
 __device__ __forceinline__ float3 out_velocity(const FtlStep& A, const float4* __restrict__ pp, int j, float3 p,
                                                float3 u) {
-  if (A.vmode != kVelDftl) return u;
+  if (FTL_NO_DFTL || A.vmode != kVelDftl) return u;
   const float4 x = pp[j];
   float3 v = f3(((p.x - x.x) - u.x) * A.inv_dt, ((p.y - x.y) - u.y) * A.inv_dt, ((p.z - x.z) - u.z) * A.inv_dt);
   const float dp = dot3(v, v);
@@ -120,8 +140,8 @@ __device__ __forceinline__ void flush_chunk(float4* __restrict__ dst, int strand
 // the lane's last point is left to the caller (its velocity is the raw
 // correction and its collision comes after the sweep).
 template <int M, bool GUARD, bool FINAL>
-__device__ __forceinline__ void sweep(const FtlStep& A, float3 (&p)[M], const float (&l)[M], int nl,
-                                      const float4* __restrict__ pp, float3 u0, float3& last_pre, float3& clast,
+__device__ __forceinline__ void sweep(const FtlStep& A, const Spheres& S, float3 (&p)[M], const LenLds& l, int nl,
+                                      const float4* __restrict__ pp, float3& last_pre, float3& clast,
                                       float4* __restrict__ stage, int strand0, int N, int lane, unsigned cm) {
   constexpr int C = Tile<M, 1, GUARD>::C;
   // Opaque per sweep: keeps the compiler from hoisting sigma * l[j] out of the
@@ -129,28 +149,27 @@ __device__ __forceinline__ void sweep(const FtlStep& A, float3 (&p)[M], const fl
   float sg = A.sigma;
   asm volatile("" : "+v"(sg));
   const float dl = A.delta;
-  float3 lead = p[0];
 #pragma unroll
   for (int j = 1; j < M; ++j) {
     // GUARD: slots j >= nl hold copies of the last point; they are swept like
     // the others (finite, never stored) so the loop keeps a fixed shape.
+    // The leader p[j-1] has not been collided yet: its collision follows.
     const float3 q = p[j];
-    const float3 d = sub3(q, lead);
-    const float s = (sg * l[j]) * rsqrt_c(dot3(d, d));
-    const float3 pn = fma3(s, d, lead);
-    p[j] = pn;
-    lead = pn;
+    ftl_point(p[j], p[j - 1], sg * l.get(j));
+    const float3 pn = p[j];
     if (FINAL) {
       const float3 c = sub3(pn, q);
       if (!GUARD || j < nl) {
-        // Point m = j-1 is final: u = delta * c_j (the root keeps its own).
+        // Point m = j-1 is final: u = delta * c_j.  The root keeps its own,
+        // p0 - x0 (:192), formed here from x0, which is still in memory.
         const int m = j - 1;
         float3 pm = p[m];
         float3 u = mul3(c, dl);
         if (m == 0) {
-          u = u0;
+          const float4 x0 = pp[0];
+          u = sub3(pm, f3(x0.x, x0.y, x0.z));
         } else {
-          collide<true>(A, pm, u, cm);
+          collide<true>(S, pm, u, cm);
         }
         p[m] = pm;
         const float3 v = out_velocity(A, pp, m, pm, u);
@@ -164,12 +183,12 @@ __device__ __forceinline__ void sweep(const FtlStep& A, float3 (&p)[M], const fl
     } else {
       if (j >= 2) {
         float3 dummy;
-        collide<false>(A, p[j - 1], dummy, cm);
+        collide<false>(S, p[j - 1], dummy, cm);
       }
       if (GUARD ? (j == nl - 1) : (j == M - 1)) {
         last_pre = pn;
         float3 dummy;
-        collide<false>(A, p[j], dummy, cm);
+        collide<false>(S, p[j], dummy, cm);
       }
     }
   }
@@ -177,18 +196,18 @@ __device__ __forceinline__ void sweep(const FtlStep& A, float3 (&p)[M], const fl
 
 // Lane-predicated collisions of one point: lanes with `on` false are left
 // untouched; `reflect` also reflects u (last iteration).
-__device__ __forceinline__ void collide_pred(const FtlStep& A, float3& p, float3& u, bool on, bool reflect,
+__device__ __forceinline__ void collide_pred(const Spheres& S, float3& p, float3& u, bool on, bool reflect,
                                              unsigned cmask) {
-#pragma unroll 1
-  for (unsigned mk = cmask; mk; mk &= mk - 1) {
-    const int k = __builtin_ctz(mk);
-    const float3 c = f3(A.sph[k][0], A.sph[k][1], A.sph[k][2]);
+#pragma unroll
+  for (int k = 0; k < FTL_NCOL; ++k) {
+    if (!has_collider(cmask, k)) continue;
+    const float3 c = f3(S.c[k][0], S.c[k][1], S.c[k][2]);
     const float3 w = sub3(p, c);
     const float dp = dot3(w, w);
-    if (on && dp < A.r2[k]) {
+    if (on && dp < S.r2[k]) {
       const float is = rsqrt_c(dp);
       const float3 n = mul3(w, is);
-      p = fma3(A.sph[k][3], n, c);
+      p = fma3(S.r[k], n, c);
       if (reflect) {
         const float d2 = 2.0f * dot3(n, u);
         u = fma3(-d2, n, u);
@@ -202,51 +221,69 @@ __device__ __forceinline__ void collide_pred(const FtlStep& A, float3& p, float3
 // iteration and also write their final velocities (lanes finish at different
 // time steps, so these go straight to memory; positions leave coalesced after
 // the pipeline drains).  State changes are selects, so no value is live across
-// the two sides of a divergent branch.
-template <int M, int G>
-__device__ __forceinline__ void sweep_pipe(const FtlStep& A, float3 (&p)[M], const float (&l)[M], float3 lead,
-                                           int g, bool active, bool fin, bool valid, const float4* __restrict__ pp,
-                                           float4* __restrict__ vp, float3 u0, float3& last_pre, float3& cfirst,
-                                           float3& clast, unsigned cm) {
+// the two sides of a divergent branch.  FINS: some lane may be in its last
+// iteration (the last G time steps); the earlier steps skip the velocity work.
+template <int M, int G, bool FINS>
+__device__ __forceinline__ void sweep_pipe(const FtlStep& A, const Spheres& S, float3 (&p)[M], const LenRegs<M>& l,
+                                           float3 lead, int g, bool active, bool fin, bool valid,
+                                           const float4* __restrict__ pp, float4* __restrict__ vp, float3 u0,
+                                           float3& last_pre, float3& cfirst, float3& clast, unsigned cm) {
   float sg = A.sigma;
   asm volatile("" : "+v"(sg));
   const float dl = A.delta;
+  if (!FINS) fin = false;
 #pragma unroll
   for (int j = 0; j < M; ++j) {
     const float3 q = p[j];
     const float3 d = sub3(q, lead);
-    const float s = (sg * l[j]) * rsqrt_c(dot3(d, d));
+    const float s = (sg * l.get(j)) * rsqrt_c(dot3(d, d));
     float3 pn = fma3(s, d, lead);
     const bool upd = active && (j != 0 || g != 0);  // lane 0's slot 0 is the root
     pn = sel3(upd, pn, q);
     p[j] = pn;
     lead = pn;
-    const float3 c = sub3(pn, q);
-    if (j == 0) cfirst = sel3(fin, c, cfirst);
-    if (j >= 1) {
-      const bool root = (j - 1 == 0) && (g == 0);
-      float3 pm = p[j - 1];
-      float3 u = mul3(c, dl);
-      if (j - 1 == 0) u = sel3(g == 0, u0, u);
-      collide_pred(A, pm, u, active && !root, fin, cm);
-      p[j - 1] = pm;
-      if (fin && valid) {
-        const float3 v = out_velocity(A, pp, j - 1, pm, u);
-        vp[j - 1] = make_float4(v.x, v.y, v.z, 0.0f);
+    if (FINS) {
+      const float3 c = sub3(pn, q);
+      if (j == 0) cfirst = sel3(fin, c, cfirst);
+      if (j >= 1) {
+        const bool root = (j - 1 == 0) && (g == 0);
+        float3 pm = p[j - 1];
+        float3 u = mul3(c, dl);
+        if (j - 1 == 0) u = sel3(g == 0, u0, u);
+        collide_pred(S, pm, u, active && !root, fin, cm);
+        p[j - 1] = pm;
+        if (fin && valid) {
+          const float3 v = out_velocity(A, pp, j - 1, pm, u);
+          vp[j - 1] = make_float4(v.x, v.y, v.z, 0.0f);
+        }
       }
+      if (j == M - 1) clast = sel3(fin, c, clast);
+    } else if (j >= 1) {
+      const bool root = (j - 1 == 0) && (g == 0);
+      float3 dummy = f3(0.f, 0.f, 0.f);
+      collide_pred(S, p[j - 1], dummy, active && !root, false, cm);
     }
     if (j == M - 1) {
       last_pre = sel3(active, pn, last_pre);
-      clast = sel3(fin, c, clast);
       float3 dummy = f3(0.f, 0.f, 0.f);
-      collide_pred(A, p[j], dummy, active && !fin, false, cm);  // deferred when final
+      collide_pred(S, p[j], dummy, active && !fin, false, cm);  // deferred when final
     }
   }
 }
 
+// Shared memory of one tile: the load/store stage, and either the rest
+// lengths (one lane per strand) or the roots' velocities (pipelined forms).
 template <int M, int G, bool GUARD>
-__device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* __restrict__ stage,
-                                         float4* __restrict__ u0s) {
+struct TileLds {
+  float4 stage[FTL_WAVE * Tile<M, G, GUARD>::C];
+  float lens[G == 1 ? FTL_WAVE * M : 1];
+  float4 u0s[G == 1 ? 1 : FTL_WAVE];
+};
+
+template <int M, int G, bool GUARD>
+__device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<M, G, GUARD>& sh) {
+  float4* __restrict__ stage = sh.stage;
+  float4* __restrict__ u0s = sh.u0s;
   constexpr int C = Tile<M, G, GUARD>::C;
   const int lane = threadIdx.x & (FTL_WAVE - 1);
   const int g = (G == 1) ? 0 : (lane % G);
@@ -258,7 +295,9 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
   const int slast = nstr - 1;
 
   float3 p[M];
-  float l[M];
+  using Len = typename std::conditional<G == 1, LenLds, LenRegs<M>>::type;
+  Len l;
+  if constexpr (G == 1) l.base = sh.lens + (threadIdx.x & (FTL_WAVE - 1));
 
   // ---- load + integrate, through LDS ------------------------------------
   // Loads are clamped into the groom (out-of-range lanes read valid data that
@@ -303,8 +342,10 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
         if (cp == 0) {
           const float3 src = root_source(A, min(strand0 + r / G, slast), x);
           pr = A.model_ident ? src : transform(A, src);
-          const float3 u = sub3(pr, f3(x.x, x.y, x.z));
-          u0s[r] = make_float4(u.x, u.y, u.z, 0.0f);
+          if (G > 1) {
+            const float3 u = sub3(pr, f3(x.x, x.y, x.z));
+            u0s[r] = make_float4(u.x, u.y, u.z, 0.0f);
+          }
         }
         stage[slot<C>(r, j)] = make_float4(pr.x, pr.y, pr.z, x.w);
         __builtin_amdgcn_sched_barrier(0);
@@ -315,12 +356,12 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
     for (int j = 0; j < C; ++j) {
       const float4 t = stage[slot<C>(lane, j)];
       p[c * C + j] = f3(t.x, t.y, t.z);
-      l[c * C + j] = t.w;
+      l.set(c * C + j, t.w);
     }
     __syncthreads();
   }
   float3 u0 = f3(0.f, 0.f, 0.f);
-  if (g == 0) {
+  if (G > 1 && g == 0) {
     const float4 t = u0s[lane];
     u0 = f3(t.x, t.y, t.z);
   }
@@ -331,7 +372,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
     float L = 0.0f;
 #pragma unroll
     for (int j = 0; j < M; ++j)
-      if (j >= (G == 1 ? 1 : 0) && (!GUARD || j < nl)) L += A.sigma * l[j];
+      if (j >= (G == 1 ? 1 : 0) && (!GUARD || j < nl)) L += A.sigma * l.get(j);
     float3 root = p[0];
     if (G > 1) {
       // strand total and root, from the lanes of the strand
@@ -344,6 +385,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
       if (__ballot((mine >> k) & 1u) != 0ull) any |= 1u << k;
     cm = __builtin_amdgcn_readfirstlane(any);
   }
+  const Spheres S = load_spheres(A, cm);
 
   // ---- constraints ------------------------------------------------------
   // Lane's own points, for the DFTL re-read of x (never stored through).
@@ -353,17 +395,17 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
   float3 cfirst = f3(0.f, 0.f, 0.f), clast = f3(0.f, 0.f, 0.f);
   float3 u;
   bool bad = false;
-  if (G == 1) {
+  if constexpr (G == 1) {
     for (int it = 0; it < K - 1; ++it)
-      sweep<M, GUARD, false>(A, p, l, nl, pp, u0, last_pre, clast, stage, strand0, N, lane, cm);
-    sweep<M, GUARD, true>(A, p, l, nl, pp, u0, last_pre, clast, stage, strand0, N, lane, cm);
+      sweep<M, GUARD, false>(A, S, p, l, nl, pp, last_pre, clast, stage, strand0, N, lane, cm);
+    sweep<M, GUARD, true>(A, S, p, l, nl, pp, last_pre, clast, stage, strand0, N, lane, cm);
     // The last point: velocity = raw correction, collisions deferred to here.
     u = clast;
 #pragma unroll
     for (int j = 1; j < M; ++j) {
       if (GUARD ? (j == nl - 1) : (j == M - 1)) {
         float3 pm = p[j];
-        collide<true>(A, pm, u, cm);
+        collide<true>(S, pm, u, cm);
         p[j] = pm;
         const float3 v = out_velocity(A, pp, j, pm, u);
         bad = !(isfinite(pm.x) && isfinite(pm.y) && isfinite(pm.z) && isfinite(v.x) && isfinite(v.y) &&
@@ -374,20 +416,27 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
     }
   } else {
     float4* __restrict__ vp = A.vel + (size_t)(valid ? strand : 0) * N + (size_t)g * M;
-    for (int t = 0; t < K + G - 1; ++t) {
+    // No lane is in its last iteration before time step K - 1.
+    for (int t = 0; t < K - 1; ++t) {
+      const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
+                           __shfl_up(last_pre.z, 1, G));
+      const float3 lead = sel3(g == 0, p[0], li);
+      sweep_pipe<M, G, false>(A, S, p, l, lead, g, t >= g, false, valid, pp, vp, u0, last_pre, cfirst, clast, cm);
+    }
+    for (int t = K - 1; t < K + G - 1; ++t) {
       const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
                            __shfl_up(last_pre.z, 1, G));
       const float3 lead = sel3(g == 0, p[0], li);
       const int it = t - g;
-      sweep_pipe<M, G>(A, p, l, lead, g, it >= 0 && it < K, it == K - 1, valid, pp, vp, u0, last_pre, cfirst,
-                       clast, cm);
+      sweep_pipe<M, G, true>(A, S, p, l, lead, g, it >= 0 && it < K, it == K - 1, valid, pp, vp, u0, last_pre,
+                             cfirst, clast, cm);
     }
     // The lane's last point: its velocity needs the next lane's correction.
     const float3 cn = f3(__shfl_down(cfirst.x, 1, G), __shfl_down(cfirst.y, 1, G),
                          __shfl_down(cfirst.z, 1, G));
     u = sel3(g == G - 1, clast, mul3(cn, A.delta));
     float3 pm = p[M - 1];
-    collide<true>(A, pm, u, cm);
+    collide<true>(S, pm, u, cm);
     p[M - 1] = pm;
     const float3 v = out_velocity(A, pp, M - 1, pm, u);
     if (valid) vp[M - 1] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -401,7 +450,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, float4* 
 #pragma unroll
     for (int j = 0; j < C; ++j) {
       const float3 q = p[c * C + j];
-      stage[slot<C>(lane, j)] = make_float4(q.x, q.y, q.z, l[c * C + j]);
+      stage[slot<C>(lane, j)] = make_float4(q.x, q.y, q.z, l.get(c * C + j));
     }
     flush_chunk<M, G, GUARD>(A.pos, strand0, nstr, N, c, stage, lane);
   }
