@@ -8,6 +8,7 @@ class), so the objects compile in parallel.
 """
 import concurrent.futures
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -17,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libftl.so")
-NCLASSES = 15
+NCLASSES = int(re.search(r"#define FTL_NCLASSES (\d+)", open(os.path.join(HERE, "csrc", "ftl_classes.h")).read()).group(1))
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

@@ -1,6 +1,7 @@
 // Launch-shape autotuner: for each point count, times the exact launch
 // classes (lanes per strand 1, 2, 4) on a synthetic groom and keeps the
-// fastest.  Results do not depend on the choice (every class performs the
+// fastest.  The measured time per control point of every class is kept too:
+// batched frames share the device among classes in proportion to it.  Results do not depend on the choice (every class performs the
 // same IEEE operations per point); only the time does.
 #include <hip/hip_runtime.h>
 
@@ -29,6 +30,17 @@ static bool load_cache(ftl_ctx* ctx, const char* path) {
   char key[32];
   std::snprintf(key, sizeof key, "\"ncu\": %d,", ctx->ncu);
   if (s.find(key) == std::string::npos) return false;
+  const size_t ct = s.find("\"class_ns_per_cp\"");
+  if (ct != std::string::npos) {
+    for (int c = 0; c < FTL_NCLASSES; ++c) {
+      std::snprintf(key, sizeof key, "\"%d\": ", c);
+      const size_t k = s.find(key, ct);
+      const size_t end = s.find('}', ct);
+      if (k == std::string::npos || k > end) continue;
+      const float v = (float)std::atof(s.c_str() + k + std::strlen(key));
+      if (v > 0.0f && v < 1e3f) ctx->cls_ns_cp[c] = v;
+    }
+  }
   const size_t at = s.find("\"lanes\"");
   if (at == std::string::npos) return false;
   int found = 0;
@@ -53,6 +65,15 @@ static void save_cache(ftl_ctx* ctx, const char* path, const std::vector<std::st
   for (int n : kTunedPoints) {
     if (!ctx->lanes[n]) continue;
     f << (first ? "" : ",") << "\n    \"" << n << "\": " << ctx->lanes[n];
+    first = false;
+  }
+  f << "\n  },\n  \"class_ns_per_cp\": {";
+  first = true;
+  for (int c = 0; c < FTL_NCLASSES; ++c) {
+    if (!(ctx->cls_ns_cp[c] > 0.0f)) continue;
+    char v[48];
+    std::snprintf(v, sizeof v, "\n    \"%d\": %.6f", c, ctx->cls_ns_cp[c]);
+    f << (first ? "" : ",") << v;
     first = false;
   }
   f << "\n  },\n  \"timings_us\": [";
@@ -123,6 +144,7 @@ extern "C" ftl_status ftl_autotune(ftl_ctx* ctx, const char* cache_path, int for
       char note[96];
       std::snprintf(note, sizeof note, "{\"npoints\": %d, \"lanes\": %d, \"us\": %.2f}", N, G, us);
       notes.push_back(note);
+      ctx->cls_ns_cp[cls] = 1000.0f * us / ((float)S * (float)N);
       if (us < best_t) {
         best_t = us;
         best_g = G;

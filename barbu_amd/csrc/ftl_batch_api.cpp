@@ -49,12 +49,26 @@ struct ftl_batch {
 
 static const int kCounterBytes = 256;  // up to 64 class counters
 
+// Time per control point of a launch class in ns for K iterations: measured
+// by the autotuner (at 8 iterations) where available, otherwise estimated.
+static double class_ns_per_cp(const ftl_ctx* ctx, int cls, int K) {
+  const FtlClassInfo& k = kFtlClasses[cls];
+  // per-lane work: M points through K + G - 1 pipeline steps, plus about
+  // three steps' worth of loads, wind and stores
+  const double steps = (double)(K + k.g + 2) / (double)(8 + k.g + 2);
+  double ns = ctx->cls_ns_cp[cls];
+  if (!(ns > 0.0)) ns = (k.guarded ? 0.028 : 0.022) * (k.g == 1 ? 1.0 : 1.0 + 0.1 * (k.g - 1));
+  return ns * steps;
+}
+
 // Plans the tiles of a set of grooms, grouped by launch class (heaviest class
-// first; within a class, grooms with more iterations first).
+// first; within a class, grooms with more iterations first), and sizes each
+// class's persistent grid.
 static ftl_status plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::vector<FtlTile>& tiles,
                              std::vector<ClassRun>& runs) {
   tiles.clear();
   runs.clear();
+  std::vector<double> work(FTL_NCLASSES, 0.0);
   for (int i = 0; i < n; ++i) {
     ftl_groom* g = grooms[i];
     if (!g) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom %d is NULL", i);
@@ -66,7 +80,11 @@ static ftl_status plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std:
     const FtlClassInfo& k = kFtlClasses[cls];
     const int spt = FTL_WAVE / k.g;
     const int cost = k.m * (g->params.iterations + k.g - 1);
-    for (int s0 = 0; s0 < g->nstrands; s0 += spt) tiles.push_back(FtlTile{i, s0, cls, cost});
+    const double ns_tile = class_ns_per_cp(ctx, cls, g->params.iterations) * (double)FTL_WAVE * k.m;
+    for (int s0 = 0; s0 < g->nstrands; s0 += spt) {
+      tiles.push_back(FtlTile{i, s0, cls, cost});
+      work[cls] += ns_tile;
+    }
   }
   std::vector<int> cls_cost(FTL_NCLASSES, -1);
   for (const FtlTile& t : tiles) cls_cost[t.cls] = std::max(cls_cost[t.cls], t.cost);
@@ -77,13 +95,20 @@ static ftl_status plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std:
     }
     return a.cost > b.cost;
   });
+  // The classes run side by side.  Each gets a share of the device's wave
+  // slots in proportion to its expected time on the whole device, so that
+  // all of them drain at about the same time.
+  double total = 0.0;
+  for (double w : work) total += w;
   for (int i = 0; i < (int)tiles.size();) {
     int j = i;
     while (j < (int)tiles.size() && tiles[j].cls == tiles[i].cls) ++j;
-    int per_cu = ftl_persist_blocks_per_cu(tiles[i].cls);
+    const int cls = tiles[i].cls;
+    int per_cu = ftl_persist_blocks_per_cu(cls);
     if (per_cu <= 0) per_cu = 4;
-    const long cap = (long)ctx->ncu * per_cu;
-    runs.push_back(ClassRun{tiles[i].cls, i, j - i, (int)std::max(1L, std::min<long>(cap, j - i))});
+    const double share = total > 0.0 ? work[cls] / total : 1.0;
+    const long slots = (long)((double)ctx->ncu * per_cu * share + 0.999);
+    runs.push_back(ClassRun{cls, i, j - i, (int)std::max(1L, std::min<long>(slots, j - i))});
     i = j;
   }
   if (runs.size() > 64) return ftl_fail(FTL_ERR_UNSUPPORTED, "too many launch classes");

@@ -6,32 +6,44 @@
 
 // X(id, points per lane, lanes per strand, guarded, waves per SIMD targeted
 // by the register allocator: the highest at which neither the plain nor the
-// persistent kernel of the class spills)
-#define FTL_CLASSES(X)     \
-  X(0, 4, 1, false, 8)     \
-  X(1, 8, 1, false, 6)     \
-  X(2, 16, 1, false, 4)    \
-  X(3, 32, 1, false, 2)    \
-  X(4, 8, 2, false, 4)     \
-  X(5, 16, 2, false, 2)    \
-  X(6, 32, 2, false, 2)    \
-  X(7, 4, 4, false, 6)     \
-  X(8, 8, 4, false, 4)     \
-  X(9, 16, 4, false, 3)    \
-  X(10, 4, 1, true, 6)     \
-  X(11, 8, 1, true, 6)     \
-  X(12, 16, 1, true, 4)    \
-  X(13, 32, 1, true, 2)    \
-  X(14, 64, 1, true, 2)
+// persistent kernel of the class spills, points per lane of a load/store
+// chunk: 8 float4 make one 128-byte line, so each line of the groom is read
+// by one request; 4 halves the LDS stage where a bigger one would cost
+// occupancy, at the price of two requests per line)
+#define FTL_CLASSES(X)        \
+  X(0, 4, 1, false, 8, 4)     \
+  X(1, 8, 1, false, 6, 4)     \
+  X(2, 16, 1, false, 4, 4)    \
+  X(3, 32, 1, false, 2, 8)    \
+  X(4, 8, 2, false, 4, 8)     \
+  X(5, 16, 2, false, 2, 8)    \
+  X(6, 32, 2, false, 2, 8)    \
+  X(7, 4, 4, false, 6, 4)     \
+  X(8, 8, 4, false, 4, 8)     \
+  X(9, 16, 4, false, 3, 8)    \
+  X(10, 4, 1, true, 6, 4)     \
+  X(11, 8, 1, true, 6, 4)     \
+  X(12, 16, 1, true, 4, 4)    \
+  X(13, 32, 1, true, 2, 4)    \
+  X(14, 64, 1, true, 2, 4)    \
+  X(15, 4, 2, false, 7, 4)
 
-#define FTL_NCLASSES 15
+#define FTL_NCLASSES 16
 
 struct FtlClassInfo {
-  int m, g, guarded, waves;
+  int m, g, guarded, waves, chunk;
 };
 
 static constexpr FtlClassInfo kFtlClasses[FTL_NCLASSES] = {
-#define FTL_CI(id, M, G, GU, W) {M, G, GU ? 1 : 0, W},
+#define FTL_CI(id, M, G, GU, W, CH) {M, G, GU ? 1 : 0, W, CH},
     FTL_CLASSES(FTL_CI)
 #undef FTL_CI
 };
+
+// Load/store chunk of the class with these template parameters.
+constexpr int ftl_chunk_of(int m, int g, bool guarded) {
+  for (int i = 0; i < FTL_NCLASSES; ++i)
+    if (kFtlClasses[i].m == m && kFtlClasses[i].g == g && (kFtlClasses[i].guarded != 0) == guarded)
+      return kFtlClasses[i].chunk < m ? kFtlClasses[i].chunk : m;
+  return m < 4 ? m : 4;
+}

@@ -8,12 +8,14 @@
 #include <vector>
 
 #include "ftl.h"
+#include "ftl_classes.h"
 #include "ftl_device.h"
 
 struct ftl_ctx {
   int device = 0;
   int ncu = 0;
   int lanes[FTL_MAX_POINTS + 1];  // tuned lanes per strand by point count (0 = default rule)
+  float cls_ns_cp[FTL_NCLASSES] = {};  // measured ns per control point of each class (0 = not measured)
   std::mutex mu;
   uint64_t bytes = 0;  // device memory held by the context's objects
   int live_objects = 0;

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void ftl_copy_kernel(const float4* __restrict_
 // ---------------------------------------------------------------------------
 // Host dispatch.
 
-#define FTL_DECL(id, M, G, GU, W)                                                                    \
+#define FTL_DECL(id, M, G, GU, W, CH)                                                                    \
   hipError_t ftl_launch_plain_##id(const FtlStep& A, hipStream_t stream);                           \
   hipError_t ftl_launch_persist_##id(const FtlStep*, const FtlTile*, int, unsigned*, int, hipStream_t); \
   int ftl_persist_occ_##id();
@@ -90,7 +90,7 @@ FTL_CLASSES(FTL_DECL)
 
 int ftl_persist_blocks_per_cu(int cls) {
   switch (cls) {
-#define FTL_OC(id, M, G, GU, W) \
+#define FTL_OC(id, M, G, GU, W, CH) \
   case id:                   \
     return ftl_persist_occ_##id();
     FTL_CLASSES(FTL_OC)
@@ -106,7 +106,7 @@ int ftl_class_guarded(int cls) { return (cls >= 0 && cls < FTL_NCLASSES) ? kFtlC
 
 hipError_t ftl_launch_plain(int cls, const FtlStep& A, hipStream_t stream) {
   switch (cls) {
-#define FTL_LP(id, M, G, GU, W) \
+#define FTL_LP(id, M, G, GU, W, CH) \
   case id:                   \
     return ftl_launch_plain_##id(A, stream);
     FTL_CLASSES(FTL_LP)
@@ -124,7 +124,7 @@ hipError_t ftl_reset_counters(unsigned* counters, int n, hipStream_t stream) {
 hipError_t ftl_launch_persist(int cls, const FtlStep* grooms, const FtlTile* tiles, int ntiles, unsigned* counter,
                               int grid, hipStream_t stream) {
   switch (cls) {
-#define FTL_LB(id, M, G, GU, W) \
+#define FTL_LB(id, M, G, GU, W, CH) \
   case id:                   \
     return ftl_launch_persist_##id(grooms, tiles, ntiles, counter, grid, stream);
     FTL_CLASSES(FTL_LB)

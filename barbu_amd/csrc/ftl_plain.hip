@@ -12,7 +12,7 @@ namespace {
 
 template <int ID>
 struct ClassOf;
-#define FTL_CO(id, M_, G_, GU_, W_)              \
+#define FTL_CO(id, M_, G_, GU_, W_, CH_)         \
   template <>                                \
   struct ClassOf<id> {                       \
     static constexpr int M = M_, G = G_, W = W_; \

@@ -36,6 +36,7 @@
 
 #include <type_traits>
 
+#include "ftl_classes.h"
 #include "ftl_device.h"
 #ifndef FTL_NO_DFTL
 #define FTL_NO_DFTL 0
@@ -53,7 +54,7 @@ enum { kRootModel = 0, kRootRest = 1, kRootSkinned = 2 };
 
 template <int M, int G, bool GUARD>
 struct Tile {
-  static constexpr int C = (M < 4) ? M : 4;  // points per lane per load/store chunk
+  static constexpr int C = ftl_chunk_of(M, G, GUARD);  // points per lane per load/store chunk
   static constexpr int SPT = FTL_WAVE / G;   // strands per tile
 };
 
@@ -285,7 +286,11 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
   float4* __restrict__ stage = sh.stage;
   float4* __restrict__ u0s = sh.u0s;
   constexpr int C = Tile<M, G, GUARD>::C;
-  const int lane = threadIdx.x & (FTL_WAVE - 1);
+  // Opaque: in the persistent kernels nothing lane-derived is hoisted out of
+  // the tile loop (it would stay live, in registers, across every tile).
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int lane = tid & (FTL_WAVE - 1);
   const int g = (G == 1) ? 0 : (lane % G);
   const int N = GUARD ? A.npoints : M * G;
   const int nl = GUARD ? A.npoints : M;  // points held by this lane (G == 1 when GUARD)
@@ -297,7 +302,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
   float3 p[M];
   using Len = typename std::conditional<G == 1, LenLds, LenRegs<M>>::type;
   Len l;
-  if constexpr (G == 1) l.base = sh.lens + (threadIdx.x & (FTL_WAVE - 1));
+  if constexpr (G == 1) l.base = sh.lens + lane;
 
   // ---- load + integrate, through LDS ------------------------------------
   // Loads are clamped into the groom (out-of-range lanes read valid data that

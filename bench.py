@@ -227,8 +227,11 @@ def bench_batched(ba, hipm, frames, warmup, comp=None):
     ntiles, grid = batch.info()
     dts = np.full(len(grooms), DT, np.float32)
     res = {}
+    # Time runs on through both legs: the plain leg continues the motion where
+    # the batched one stopped (a jump back in time would teleport every root).
+    frame0 = 0
     for mode in ("batched", "plain"):
-        for k in range(warmup):
+        for k in range(frame0, frame0 + warmup):
             c3_frame_inputs(comp, grooms, (k + 1) * float(DT))
             if mode == "batched":
                 batch.launch(dts, stream.h)
@@ -238,7 +241,7 @@ def bench_batched(ba, hipm, frames, warmup, comp=None):
         hipm.device_synchronize()
         w0 = time.perf_counter()
         e0.record(stream.h)
-        for k in range(warmup, warmup + frames):
+        for k in range(frame0 + warmup, frame0 + warmup + frames):
             c3_frame_inputs(comp, grooms, (k + 1) * float(DT))
             if mode == "batched":
                 batch.launch(dts, stream.h)
@@ -249,6 +252,7 @@ def bench_batched(ba, hipm, frames, warmup, comp=None):
         hipm.device_synchronize()
         wall = time.perf_counter() - w0
         res[mode] = dict(wall_s=wall, gpu_ms=e0.elapsed_ms(e1))
+        frame0 += warmup + frames
     for g in grooms:
         p, v = g.download()
         assert np.isfinite(p).all() and np.isfinite(v).all(), "non-finite state after the batched benchmark"
