@@ -11,9 +11,11 @@ import numpy as np
 from . import _ffi
 from ._ffi import (FTL_ROOT_MODEL, FTL_ROOT_REST, FTL_ROOT_SKINNED, FTL_VELOCITY_DFTL,
                    FTL_VELOCITY_SHIPPED, FTL_MAX_COLLIDERS, FTL_MAX_POINTS, FTL_WIND_GRID,
+                   FTL_OK, FTL_ERR_INVALID_ARGUMENT, FTL_ERR_DEVICE, FTL_ERR_OUT_OF_MEMORY, FTL_ERR_BUDGET,
+                   FTL_ERR_NOT_INITIALIZED, FTL_ERR_NONFINITE, FTL_ERR_UNSUPPORTED, FTL_ERR_BUSY,
                    FtlError, Params, check, lib)
 
-__all__ = ["Context", "Groom", "Skin", "Batch", "Params", "reference_params", "FtlError",
+__all__ = ["Context", "Groom", "Skin", "Batch", "Server", "make_requests", "set_request", "Params", "reference_params", "FtlError",
            "sphere_groom", "reference_groom", "skinned_scene", "bones", "wind_table",
            "FTL_ROOT_MODEL", "FTL_ROOT_REST", "FTL_ROOT_SKINNED", "FTL_VELOCITY_SHIPPED",
            "FTL_VELOCITY_DFTL", "Hair"]
@@ -264,6 +266,131 @@ def batch_step(ctx, grooms, dt, stream=None):
     arr = (C.c_void_p * len(grooms))(*[g.handle.value for g in grooms])
     dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (len(grooms),)))
     check(lib().ftl_batch_step(ctx.handle, arr, len(grooms), _ptr(dt), stream), "ftl_batch_step")
+
+
+# --- serving loop ---------------------------------------------------------------
+
+# numpy layout of ftl_step_request (144 bytes, no padding)
+REQUEST_DTYPE = np.dtype([("dt", "<f4"), ("time", "<f4"), ("has_transform", "<i4"), ("model", "<f4", (16,)),
+                          ("ncolliders", "<i4"), ("spheres", "<f4", (16,))])
+assert REQUEST_DTYPE.itemsize == C.sizeof(_ffi.StepRequest)
+
+
+def make_requests(n, dt=1.0 / 60.0, time=0.0):
+    """n step requests that keep every creature's transform and colliders."""
+    r = np.zeros(n, REQUEST_DTYPE)
+    r["dt"] = dt
+    r["time"] = time
+    r["ncolliders"] = -1
+    return r
+
+
+def set_request(r, dt=None, time=None, model=None, spheres=None):
+    """Fills one request (a REQUEST_DTYPE record); model is a row-major 4x4 (as
+    Groom.set_root_transform), spheres an (k, 4) array."""
+    if dt is not None:
+        r["dt"] = dt
+    if time is not None:
+        r["time"] = time
+    if model is not None:
+        r["has_transform"] = 1
+        r["model"] = np.asarray(model, np.float32).reshape(4, 4).T.reshape(16)
+    if spheres is not None:
+        sp = np.asarray(spheres, np.float32).reshape(-1, 4)
+        r["ncolliders"] = sp.shape[0]
+        r["spheres"] = 0.0
+        r["spheres"][: sp.size] = sp.reshape(-1)
+    return r
+
+
+class Server:
+    """Resident creatures stepped by continuous batching (ftl_server_*)."""
+
+    def __init__(self, ctx, hbm_budget_bytes=0, batch_cap_bytes=0, max_inflight=0):
+        cfg = _ffi.ServerConfig(int(hbm_budget_bytes), int(batch_cap_bytes), int(max_inflight))
+        self._h = C.c_void_p()
+        self.ctx = ctx
+        self.shapes = {}
+        check(lib().ftl_server_create(ctx.handle, C.byref(cfg), C.byref(self._h)), "ftl_server_create")
+
+    def add(self, pos, vel=None, rest_roots=None, params=None):
+        """Returns (id, resident)."""
+        pos = _f32(pos)
+        S, N = int(pos.shape[0]), int(pos.shape[1])
+        pos = pos.reshape(S, N, 4)
+        vel = None if vel is None else _f32(vel, pos.shape)
+        roots = None if rest_roots is None else _f32(rest_roots, (S, 4))
+        d = _ffi.GroomDesc(S, N, pos.ctypes.data, None if vel is None else vel.ctypes.data,
+                           None if roots is None else roots.ctypes.data)
+        p = params if params is not None else reference_params()
+        cid, res = C.c_int(0), C.c_int(0)
+        check(lib().ftl_server_add(self._h, C.byref(d), C.byref(p), C.byref(cid), C.byref(res)), "ftl_server_add")
+        self.shapes[cid.value] = (S, N)
+        return cid.value, bool(res.value)
+
+    def remove(self, cid):
+        check(lib().ftl_server_remove(self._h, int(cid)), "ftl_server_remove")
+        self.shapes.pop(cid, None)
+
+    def resident(self, cid):
+        r = C.c_int(0)
+        check(lib().ftl_server_resident(self._h, int(cid), C.byref(r)), "ftl_server_resident")
+        return bool(r.value)
+
+    def submit(self, ids, reqs):
+        """Queues requests (REQUEST_DTYPE records) for creatures ids; returns tickets."""
+        ids = np.ascontiguousarray(np.atleast_1d(ids), np.int32)
+        reqs = np.ascontiguousarray(np.atleast_1d(reqs), REQUEST_DTYPE)
+        if ids.shape[0] != reqs.shape[0]:
+            raise ValueError("ids and requests differ in length")
+        tickets = np.zeros(ids.shape[0], np.uint64)
+        check(lib().ftl_server_submit(self._h, int(ids.shape[0]), ids.ctypes.data, reqs.ctypes.data,
+                                      tickets.ctypes.data), "ftl_server_submit")
+        return tickets
+
+    def step(self, cid, dt, time=0.0, model=None, spheres=None):
+        r = make_requests(1, dt, time)
+        set_request(r[0], model=model, spheres=spheres)
+        return int(self.submit([cid], r)[0])
+
+    def wait(self, ticket, timeout_ms=-1):
+        check(lib().ftl_server_wait(self._h, int(ticket), int(timeout_ms)), "ftl_server_wait")
+
+    def drain(self, timeout_ms=-1):
+        check(lib().ftl_server_drain(self._h, int(timeout_ms)), "ftl_server_drain")
+
+    def download(self, cid):
+        if cid not in self.shapes:  # let the library report the unknown creature
+            check(lib().ftl_server_download(self._h, int(cid), None, None), "ftl_server_download")
+        S, N = self.shapes[cid]
+        pos = np.empty((S, N, 4), np.float32)
+        vel = np.empty_like(pos)
+        check(lib().ftl_server_download(self._h, int(cid), _ptr(pos), _ptr(vel)), "ftl_server_download")
+        return pos, vel
+
+    def status(self, cid):
+        nf = C.c_int(0)
+        check(lib().ftl_server_status(self._h, int(cid), C.byref(nf)), "ftl_server_status")
+        return bool(nf.value)
+
+    def stats(self):
+        st = _ffi.ServerStats()
+        check(lib().ftl_server_get_stats(self._h, C.byref(st)), "ftl_server_get_stats")
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+    def reset_stats(self):
+        check(lib().ftl_server_reset_stats(self._h), "ftl_server_reset_stats")
+
+    def close(self):
+        if self._h:
+            lib().ftl_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # --- scenario generators (CPU) ----------------------------------------------

@@ -90,22 +90,41 @@ class ScnSkinned(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_uint64),
+        ("batch_cap_bytes", C.c_uint64),
+        ("max_inflight", C.c_int),
+    ]
+
+
+class StepRequest(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float),
+        ("time", C.c_float),
+        ("has_transform", C.c_int),
+        ("model", C.c_float * 16),
+        ("ncolliders", C.c_int),
+        ("spheres", C.c_float * 16),
+    ]
+
+
 class ServerStats(C.Structure):
     _fields_ = [
-        ("requests", C.c_uint64),
+        ("submitted", C.c_uint64),
         ("completed", C.c_uint64),
-        ("rejected", C.c_uint64),
         ("batches", C.c_uint64),
-        ("points_stepped", C.c_uint64),
-        ("bytes_stepped", C.c_uint64),
-        ("wall_seconds", C.c_double),
-        ("gpu_busy_seconds", C.c_double),
-        ("latency_p50_ms", C.c_double),
-        ("latency_p99_ms", C.c_double),
-        ("latency_max_ms", C.c_double),
-        ("latency_mean_ms", C.c_double),
-        ("mean_batch_grooms", C.c_double),
-        ("max_batch_bytes", C.c_double),
+        ("cps_stepped", C.c_uint64),
+        ("p50_us", C.c_double),
+        ("p99_us", C.c_double),
+        ("max_us", C.c_double),
+        ("mean_us", C.c_double),
+        ("mean_batch_requests", C.c_double),
+        ("busy_s", C.c_double),
+        ("span_s", C.c_double),
+        ("resident_bytes", C.c_uint64),
+        ("resident", C.c_int),
+        ("waiting", C.c_int),
     ]
 
 
@@ -161,6 +180,18 @@ SIGNATURES = {
     "ftl_scn_skinned_mesh": (st, [C.POINTER(ScnSkinned), vp, vp, vp, vp]),
     "ftl_scn_skinned_groom": (st, [C.POINTER(ScnSkinned), vp, vp, vp, vp, vp, vp]),
     "ftl_scn_bones": (st, [C.POINTER(ScnSkinned), C.c_double, vp]),
+    "ftl_server_create": (st, [vp, C.POINTER(ServerConfig), pp]),
+    "ftl_server_destroy": (None, [vp]),
+    "ftl_server_add": (st, [vp, C.POINTER(GroomDesc), C.POINTER(Params), ip, ip]),
+    "ftl_server_remove": (st, [vp, C.c_int]),
+    "ftl_server_resident": (st, [vp, C.c_int, ip]),
+    "ftl_server_submit": (st, [vp, C.c_int, vp, vp, vp]),
+    "ftl_server_wait": (st, [vp, C.c_uint64, C.c_int]),
+    "ftl_server_drain": (st, [vp, C.c_int]),
+    "ftl_server_download": (st, [vp, C.c_int, vp, vp]),
+    "ftl_server_status": (st, [vp, C.c_int, ip]),
+    "ftl_server_get_stats": (st, [vp, C.POINTER(ServerStats)]),
+    "ftl_server_reset_stats": (st, [vp]),
 }
 
 _lib = None

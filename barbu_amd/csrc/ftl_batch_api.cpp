@@ -23,11 +23,6 @@
 #include "ftl_internal.h"
 #include "ftl_launch.h"
 
-// One persistent launch: the tiles [first, first + count) of one class.
-struct ClassRun {
-  int cls, first, count, grid;
-};
-
 struct ftl_batch {
   ftl_ctx* ctx = nullptr;
   std::vector<ftl_groom*> grooms;
@@ -47,7 +42,6 @@ struct ftl_batch {
   int cur = 0;
 };
 
-static const int kCounterBytes = 256;  // up to 64 class counters
 
 // Time per control point of a launch class in ns for K iterations: measured
 // by the autotuner (at 8 iterations) where available, otherwise estimated.
@@ -63,53 +57,71 @@ static double class_ns_per_cp(const ftl_ctx* ctx, int cls, int K) {
 
 // Plans the tiles of a set of grooms, grouped by launch class (heaviest class
 // first; within a class, grooms with more iterations first), and sizes each
-// class's persistent grid.
-static ftl_status plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::vector<FtlTile>& tiles,
-                             std::vector<ClassRun>& runs) {
+// class's persistent grid.  Linear in the number of tiles (only the grooms are
+// sorted): the serving loop plans a batch per launch.
+ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::vector<FtlTile>& tiles,
+                          std::vector<ClassRun>& runs) {
   tiles.clear();
   runs.clear();
+  struct GroomPlan {
+    int idx, cls, cost, ntiles;
+    double ns_tile;
+  };
+  std::vector<GroomPlan> plan;
+  plan.reserve(n);
   std::vector<double> work(FTL_NCLASSES, 0.0);
+  std::vector<int> cls_cost(FTL_NCLASSES, -1);
+  size_t total_tiles = 0;
   for (int i = 0; i < n; ++i) {
     ftl_groom* g = grooms[i];
     if (!g) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom %d is NULL", i);
     if (g->ctx != ctx) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom %d belongs to another context", i);
-    for (int j = 0; j < i; ++j)
-      if (grooms[j] == g) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom %d appears twice in the batch", i);
     const int cls = ftl_class_for(ctx, g->npoints, g->lanes, 32);
     if (cls < 0) return ftl_fail(FTL_ERR_UNSUPPORTED, "no batched launch class for npoints=%d", g->npoints);
     const FtlClassInfo& k = kFtlClasses[cls];
     const int spt = FTL_WAVE / k.g;
+    const int nt = (g->nstrands + spt - 1) / spt;
     const int cost = k.m * (g->params.iterations + k.g - 1);
     const double ns_tile = class_ns_per_cp(ctx, cls, g->params.iterations) * (double)FTL_WAVE * k.m;
-    for (int s0 = 0; s0 < g->nstrands; s0 += spt) {
-      tiles.push_back(FtlTile{i, s0, cls, cost});
-      work[cls] += ns_tile;
-    }
+    plan.push_back(GroomPlan{i, cls, cost, nt, ns_tile});
+    work[cls] += ns_tile * nt;
+    cls_cost[cls] = std::max(cls_cost[cls], cost);
+    total_tiles += (size_t)nt;
   }
-  std::vector<int> cls_cost(FTL_NCLASSES, -1);
-  for (const FtlTile& t : tiles) cls_cost[t.cls] = std::max(cls_cost[t.cls], t.cost);
-  std::stable_sort(tiles.begin(), tiles.end(), [&](const FtlTile& a, const FtlTile& b) {
+  if (total_tiles > (size_t)INT32_MAX) return ftl_fail(FTL_ERR_UNSUPPORTED, "too many tiles in one batch");
+  {
+    std::vector<ftl_groom*> seen(grooms, grooms + n);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+      return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "a groom appears twice in the batch");
+  }
+  std::stable_sort(plan.begin(), plan.end(), [&](const GroomPlan& a, const GroomPlan& b) {
     if (a.cls != b.cls) {
       if (cls_cost[a.cls] != cls_cost[b.cls]) return cls_cost[a.cls] > cls_cost[b.cls];
       return a.cls < b.cls;
     }
     return a.cost > b.cost;
   });
+  tiles.reserve(total_tiles);
   // The classes run side by side.  Each gets a share of the device's wave
   // slots in proportion to its expected time on the whole device, so that
   // all of them drain at about the same time.
   double total = 0.0;
   for (double w : work) total += w;
-  for (int i = 0; i < (int)tiles.size();) {
-    int j = i;
-    while (j < (int)tiles.size() && tiles[j].cls == tiles[i].cls) ++j;
-    const int cls = tiles[i].cls;
+  for (size_t i = 0; i < plan.size();) {
+    const int cls = plan[i].cls;
+    const int first = (int)tiles.size();
+    for (; i < plan.size() && plan[i].cls == cls; ++i) {
+      const GroomPlan& gp = plan[i];
+      const int spt = FTL_WAVE / kFtlClasses[cls].g;
+      for (int t = 0; t < gp.ntiles; ++t) tiles.push_back(FtlTile{gp.idx, t * spt, cls, gp.cost});
+    }
+    const int count = (int)tiles.size() - first;
     int per_cu = ftl_persist_blocks_per_cu(cls);
     if (per_cu <= 0) per_cu = 4;
     const double share = total > 0.0 ? work[cls] / total : 1.0;
     const long slots = (long)((double)ctx->ncu * per_cu * share + 0.999);
-    runs.push_back(ClassRun{cls, i, j - i, (int)std::max(1L, std::min<long>(slots, j - i))});
-    i = j;
+    runs.push_back(ClassRun{cls, first, count, (int)std::max(1L, std::min<long>(slots, count))});
   }
   if (runs.size() > 64) return ftl_fail(FTL_ERR_UNSUPPORTED, "too many launch classes");
   return FTL_OK;
@@ -118,7 +130,7 @@ static ftl_status plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std:
 // Enqueues the counter reset and one persistent kernel per class on s.  With
 // side streams (graph capture) the classes after the first are forked onto
 // them and joined back, so they become parallel graph branches.
-static hipError_t enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_steps, const FtlTile* d_tiles,
+hipError_t ftl_enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_steps, const FtlTile* d_tiles,
                                unsigned* d_counter, hipStream_t s, const std::vector<hipStream_t>* side,
                                const std::vector<hipEvent_t>* side_ev, hipEvent_t fork) {
   hipError_t e = ftl_reset_counters(d_counter, (int)runs.size(), s);
@@ -145,7 +157,7 @@ extern "C" ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int
   if (n == 0) return FTL_OK;
   std::vector<FtlTile> tiles;
   std::vector<ClassRun> runs;
-  ftl_status st = plan_tiles(ctx, grooms, n, tiles, runs);
+  ftl_status st = ftl_plan_tiles(ctx, grooms, n, tiles, runs);
   if (st != FTL_OK) return st;
   std::vector<FtlStep> steps(n);
   for (int i = 0; i < n; ++i)
@@ -176,7 +188,7 @@ extern "C" ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int
     return ftl_hip_fail(e, "hipMemcpy batch descriptors");
   if ((e = hipMemcpy(ctx->scratch_tiles, tiles.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
     return ftl_hip_fail(e, "hipMemcpy batch tiles");
-  if ((e = enqueue_runs(runs, (const FtlStep*)ctx->scratch_steps, (const FtlTile*)ctx->scratch_tiles,
+  if ((e = ftl_enqueue_runs(runs, (const FtlStep*)ctx->scratch_steps, (const FtlTile*)ctx->scratch_tiles,
                         ctx->scratch_counter, (hipStream_t)stream, nullptr, nullptr, nullptr)) != hipSuccess)
     return ftl_hip_fail(e, "batched step launch");
   return FTL_OK;
@@ -206,7 +218,7 @@ extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, i
   *out = nullptr;
   std::vector<FtlTile> tiles;
   std::vector<ClassRun> runs;
-  ftl_status st = plan_tiles(ctx, grooms, n, tiles, runs);
+  ftl_status st = ftl_plan_tiles(ctx, grooms, n, tiles, runs);
   if (st != FTL_OK) return st;
   hipError_t e;
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
@@ -251,7 +263,7 @@ extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, i
     if ((e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return bail(ftl_hip_fail(e, "hipStreamBeginCapture"));
     hipError_t e1 = hipMemcpyAsync(b->d_steps[k], b->h_steps[k], sb, hipMemcpyHostToDevice, b->cap);
-    hipError_t e2 = e1 == hipSuccess ? enqueue_runs(b->runs, b->d_steps[k], b->d_tiles, b->d_counter[k], b->cap,
+    hipError_t e2 = e1 == hipSuccess ? ftl_enqueue_runs(b->runs, b->d_steps[k], b->d_tiles, b->d_counter[k], b->cap,
                                                     &b->side, &b->side_ev, b->fork)
                                      : hipSuccess;
     hipError_t e3 = hipStreamEndCapture(b->cap, &b->graph[k]);
