@@ -370,6 +370,57 @@ def bench_served(ba, hipm, seconds, warm, ncreatures=256, seed=11, strand_scale=
                 device_busy_frac=st["busy_s"] / st["span_s"] if st["span_s"] > 0 else 0.0)
 
 
+def c5_spheres(t):
+    """Four colliders of the skinned workload at time t: the head (r=1 at the
+    origin) and three spheres orbiting it."""
+    return np.array([[0.0, 0.0, 0.0, 1.0],
+                     [1.3 * np.cos(t), 0.2, 1.3 * np.sin(t), 0.35],
+                     [-1.2, -0.5 + 0.2 * np.sin(2 * t), 0.3, 0.4],
+                     [0.3 * np.sin(t), 1.25, -1.1 * np.cos(t), 0.3]], np.float32)
+
+
+def bench_skinned(ba, hipm, steps, warmup, S=4_194_304, N=32):
+    """Skinned roots: 4,194,304 strands x 32 points bound to a 100k-triangle
+    mesh skinned by 64 bones (4 influences, dual quaternions, smooth random
+    rotations up to 30 degrees), 4 colliders; one skinning pass + one step per
+    frame, 120 frames."""
+    sc = ba.skinned_scene(S, N, ntris=100_000, nbones=64, radius=1.0, length=0.2, jitter=0.05, seed=5)
+    ctx = context(ba)
+    skin = ba.Skin(ctx, sc["rest"], sc["joints"], sc["weights"], sc["tris"], sc["nbones"])
+    g = ba.Groom(ctx, sc["pos"], sc["vel"])
+    g.bind_skin(skin, sc["tri"], sc["bary"])
+    g.set_params(ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0, iterations=8))
+    nverts = sc["rest"].shape[0]
+    del sc["pos"], sc["vel"]
+    stream = hipm.Stream()
+    e0, e1 = hipm.Event(), hipm.Event()
+
+    def frame(k):
+        t = (k + 1) * float(DT)
+        skin.set_bones(ba.bones(sc, t))
+        skin.update(stream.h)
+        g.set_colliders(c5_spheres(t))
+        g.step(DT, stream.h)
+
+    for k in range(warmup):
+        frame(k)
+    hipm.device_synchronize()
+    w0 = time.perf_counter()
+    e0.record(stream.h)
+    for k in range(warmup, warmup + steps):
+        frame(k)
+    e1.record(stream.h)
+    hipm.device_synchronize()
+    wall = time.perf_counter() - w0
+    gpu_ms = e0.elapsed_ms(e1)
+    nonfinite = g.status()
+    lanes = ctx.lanes_for(N)
+    g.close()
+    skin.close()
+    ctx.close()
+    return dict(S=S, N=N, nverts=nverts, wall_s=wall, gpu_ms=gpu_ms, lanes=lanes, nonfinite=bool(nonfinite))
+
+
 def load_traffic(workload):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -386,8 +437,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--workload", default="all", choices=["single", "batched", "served", "all"])
+    ap.add_argument("--workload", default="all", choices=["single", "batched", "served", "skinned", "all"])
     ap.add_argument("--served-seconds", type=float, default=4.0)
+    ap.add_argument("--skinned-steps", type=int, default=120)
     ap.add_argument("--batched-frames", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -461,6 +513,21 @@ def main():
         sv20 = bench_served(ba, hipm, args.served_seconds, 1.0, strand_scale=1.0 / 18.0)
         sv20["workload"] = "as served, strand counts scaled by 1/18 (about 20M control points)"
         out["served_20m"] = sv20
+    if args.workload in ("skinned", "all"):
+        log("skinned: 4,194,304 x 32, 100k-triangle mesh, 64 bones, 4 colliders, %d frames" % args.skinned_steps)
+        k = bench_skinned(ba, hipm, args.skinned_steps, 5)
+        kcps = k["S"] * k["N"]
+        kgb = kcps * BYTES_PER_CP / (k["gpu_ms"] / 1000.0 / args.skinned_steps) / 1e9
+        out["skinned"] = {
+            "workload": "4,194,304 strands x 32 points, roots bound to a 100k-triangle mesh skinned by 64 bones "
+                        "(4 influences, dual quaternions, rotations <= 30 deg), 4 colliders, dt=1/60",
+            "control_points": kcps, "mesh_vertices": k["nverts"], "frames": args.skinned_steps,
+            "value": kcps * args.skinned_steps / k["wall_s"], "unit": "CP updates/s",
+            "ms_per_frame": 1000.0 * k["wall_s"] / args.skinned_steps,
+            "gpu_ms_per_frame": k["gpu_ms"] / args.skinned_steps,
+            "achieved_gbs": kgb, "frac_hbm_peak": kgb / HBM_PEAK_GBS, "lanes_per_strand": k["lanes"],
+            "nonfinite": k["nonfinite"],
+        }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores)")
         out["cpu_baseline"] = cpu_baseline(ba, args.cpu_seconds)
