@@ -1,11 +1,16 @@
 """Summaries of rocprofv3 SQLite output (``*_results.db``) as short text.
 
     python -m barbu_amd.rocprof_summary trace <results.db>   per-kernel durations
+    python -m barbu_amd.rocprof_summary trace-ids <results.db>
+        the same per code object kernel (kernels of one name built for
+        different launch classes are told apart by id, VGPRs and LDS)
     python -m barbu_amd.rocprof_summary pmc <results.db>     per-kernel counter means
-    python -m barbu_amd.rocprof_summary traffic <fetch.db> <write.db> <copy_bytes>
+    python -m barbu_amd.rocprof_summary traffic <fetch.db> <write.db> <copy_bytes> [<json> <workload> <kernel>]
         HBM bytes per dispatch of each kernel from FETCH_SIZE / WRITE_SIZE runs,
         scaled by what the counters read for the plain copy kernel
-        (ftl_copy_kernel), which reads and writes copy_bytes.
+        (ftl_copy_kernel), which reads and writes copy_bytes.  With <json>,
+        also stores read + write bytes per dispatch of <kernel> under
+        <workload> in that JSON file (bench.py reports it as roofline.traffic).
 """
 import sqlite3
 import sys
@@ -30,6 +35,21 @@ def trace(db):
     for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
         out.append("%-34s %7d %12.1f %10.2f %10.2f %10.2f %6.2f" % (
             k, len(v), sum(v), sum(v) / len(v), min(v), max(v), 100.0 * sum(v) / total))
+    return "\n".join(out)
+
+
+def trace_ids(db):
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, kernel_id, vgpr_count, lds_size, duration from kernels").fetchall()
+    agg = defaultdict(list)
+    for name, kid, vg, lds, d in rows:
+        agg[(short(name), kid, vg, lds)].append(d / 1000.0)
+    total = sum(sum(v) for v in agg.values())
+    out = ["%-30s %4s %5s %6s %7s %12s %10s %10s %10s %6s" % ("kernel", "id", "vgpr", "lds", "calls", "total_us",
+                                                               "avg_us", "min_us", "max_us", "%")]
+    for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        out.append("%-30s %4d %5d %6d %7d %12.1f %10.2f %10.2f %10.2f %6.2f" % (
+            k[0], k[1], k[2], k[3], len(v), sum(v), sum(v) / len(v), min(v), max(v), 100.0 * sum(v) / total))
     return "\n".join(out)
 
 
@@ -74,10 +94,25 @@ def traffic(fetch_db, write_db, copy_bytes):
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
         print(trace(argv[1]))
+    elif len(argv) >= 2 and argv[0] == "trace-ids":
+        print(trace_ids(argv[1]))
     elif len(argv) >= 2 and argv[0] == "pmc":
         print(pmc(argv[1]))
     elif len(argv) >= 4 and argv[0] == "traffic":
-        print(traffic(argv[1], argv[2], float(argv[3]))[0])
+        text, res = traffic(argv[1], argv[2], float(argv[3]))
+        print(text)
+        if len(argv) >= 7:
+            import json
+            import os
+            path, workload, kernel = argv[4], argv[5], argv[6]
+            data = {}
+            if os.path.exists(path):
+                with open(path) as f:
+                    data = json.load(f)
+            rb, wb = res[kernel]
+            data[workload] = {"kernel": kernel, "read_bytes": rb, "write_bytes": wb, "bytes": rb + wb}
+            with open(path, "w") as f:
+                json.dump(data, f, indent=1)
     else:
         print(__doc__)
         return 2

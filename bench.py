@@ -422,11 +422,15 @@ def bench_skinned(ba, hipm, steps, warmup, S=4_194_304, N=32):
 
 
 def load_traffic(workload):
+    """HBM bytes per launch of the workload's kernel, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/traffic.json (None if
+    not profiled)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(workload)
-    except (OSError, ValueError):
+            rec = json.load(f).get(workload)
+        return None if rec is None else float(rec["bytes"])
+    except (OSError, ValueError, KeyError, TypeError):
         return None
 
 
