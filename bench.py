@@ -140,9 +140,34 @@ def cpu_baseline(ba, seconds_target=12.0):
         el = time.perf_counter() - t0
         if el >= seconds_target or k >= 2000:
             break
+    # single thread, same sample, about 2 s
+    done1, k1, t1 = 0, 0, time.perf_counter()
+    while True:
+        t = float(k + k1 + 1) * float(DT)
+        m, sph = c2_frame(t)
+        op = oracle_params_for(params, N, DT, spheres=[tuple(x) for x in sph], model=m, root_mode=1, time=t,
+                               wind=wind)
+        orc.step_f32(op, pos, vel, roots, nthreads=1)
+        done1 += S * N
+        k1 += 1
+        el1 = time.perf_counter() - t1
+        if el1 >= seconds_target / 6.0:
+            break
+    # the CPU configuration: 4,096 x 16, one sphere, 100 steps
+    c1 = {}
+    for nt in (1, threads):
+        p1, v1, _ = ba.sphere_groom(4096, 16, radius=1.0, length=0.2, jitter=0.05, seed=1, coherent=False)
+        op = oracle_params_for(ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0), 16, DT,
+                               spheres=[(0.0, 0.0, 0.0, 1.0)])
+        c0 = time.perf_counter()
+        for _ in range(100):
+            orc.step_f32(op, p1, v1, nthreads=nt)
+        c1["threads_%d_s" % nt] = time.perf_counter() - c0
+        c1["threads_%d_cps" % nt] = 4096 * 16 * 100 / c1["threads_%d_s" % nt]
     return dict(value=done / el, unit="CP updates/s", cores=threads, kind="port",
                 sample="%d strands x %d points (1/8 of the single-groom workload), %d steps, %.1f s, "
-                       "OpenMP over strands" % (S, N, k, el))
+                       "OpenMP over strands" % (S, N, k, el),
+                single_thread_value=done1 / el1, c1_4096x16_100_steps=c1)
 
 
 def c3_grooms(seed=3, ngrooms=64):

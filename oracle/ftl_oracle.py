@@ -121,6 +121,8 @@ def make_params(npoints, dt, *, iterations=8, velocity_mode=0, root_mode=0, leng
 def step_f32(P, pos, vel, roots=None, nthreads=1):
     """One step in place on float32 arrays [S, N, 4]."""
     assert pos.dtype == np.float32 and vel.dtype == np.float32 and pos.flags.c_contiguous
+    if roots is not None:
+        roots = np.ascontiguousarray(roots, np.float32)
     S = pos.shape[0]
     if nthreads > 1:
         r = lib().oracle_step_f32_mt(C.byref(P), S, _p(pos), _p(vel), _p(roots), int(nthreads))
@@ -133,6 +135,8 @@ def step_f32(P, pos, vel, roots=None, nthreads=1):
 def step_f64(P, pos, vel, roots=None):
     """One step in place on float64 arrays [S, N, 4] (same formulas in double)."""
     assert pos.dtype == np.float64 and vel.dtype == np.float64 and pos.flags.c_contiguous
+    if roots is not None:
+        roots = np.ascontiguousarray(roots, np.float64)  # the FP64 instantiation reads double roots
     r = lib().oracle_step_f64(C.byref(P), pos.shape[0], _p(pos), _p(vel), _p(roots))
     if r != 0:
         raise ValueError("oracle_step_f64 rejected its arguments")
