@@ -203,8 +203,9 @@ ftl_status ftl_groom_bind_skin(ftl_groom* g, ftl_skin* s, const int* tri, const 
 
 /* -- batched steps --------------------------------------------------------- */
 
-/* Steps n grooms of one context in one persistent launch (one kernel for all
-   point counts).  dt: one value per groom.  For repeated frames over the same
+/* Steps n grooms of one context as one batch: the grooms are cut into 64-lane
+   tiles, and one persistent kernel per launch class present pulls its tiles
+   (the classes run side by side).  dt: one value per groom.  For repeated frames over the same
    grooms, ftl_batch_create / ftl_batch_launch avoid re-planning. */
 ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int n, const float* dt,
                           void* stream);
