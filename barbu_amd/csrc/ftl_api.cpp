@@ -163,7 +163,6 @@ extern "C" ftl_status ftl_shutdown(ftl_ctx* ctx) {
   if (!ctx) return FTL_OK;
   if (ctx->scratch_steps) ftl_dev_free(ctx, ctx->scratch_steps, ctx->scratch_steps_bytes);
   if (ctx->scratch_tiles) ftl_dev_free(ctx, ctx->scratch_tiles, ctx->scratch_tiles_bytes);
-  if (ctx->scratch_counter) ftl_dev_free(ctx, ctx->scratch_counter, 256);
   if (ctx->live_objects != 0) {
     // Leaked handles keep their memory; the context itself is still released.
     ftl_fail(FTL_ERR_INVALID_ARGUMENT, "ftl_shutdown: %d objects still alive", ctx->live_objects);

@@ -4,14 +4,13 @@
 //
 // A batch is planned once: each groom is cut into 64-lane tiles of its launch
 // class (points per lane x lanes per strand), and the tiles are grouped by
-// class.  Every class has its own persistent kernel, compiled with the
-// register budget of that class (one kernel for all classes would carry the
-// register count of the heaviest one).  A frame is [upload the per-groom
-// descriptors from pinned memory -> reset the tile counters -> one persistent
-// kernel per class present, as parallel branches], captured once into a HIP
-// graph: one graph launch steps every groom.  Two graphs with their own pinned
-// descriptor buffers alternate, so the host prepares frame k+1 while frame k
-// runs.
+// class.  Every class has its own tile-list kernel, compiled with the register
+// budget of that class (one kernel for all classes would carry the register
+// count of the heaviest one).  A frame is [upload the per-groom descriptors
+// from pinned memory -> one tile-list kernel per class present, as parallel
+// branches], captured once into a HIP graph: one graph launch steps every
+// groom.  Two graphs with their own pinned descriptor buffers alternate, so
+// the host prepares frame k+1 while frame k runs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,7 +30,6 @@ struct ftl_batch {
   FtlTile* d_tiles = nullptr;
   FtlStep* d_steps[2] = {nullptr, nullptr};
   FtlStep* h_steps[2] = {nullptr, nullptr};
-  unsigned* d_counter[2] = {nullptr, nullptr};
   hipStream_t cap = nullptr;
   std::vector<hipStream_t> side;
   std::vector<hipEvent_t> side_ev;
@@ -55,10 +53,10 @@ static double class_ns_per_cp(const ftl_ctx* ctx, int cls, int K) {
   return ns * steps;
 }
 
-// Plans the tiles of a set of grooms, grouped by launch class (heaviest class
-// first; within a class, grooms with more iterations first), and sizes each
-// class's persistent grid.  Linear in the number of tiles (only the grooms are
-// sorted): the serving loop plans a batch per launch.
+// Plans the tiles of a set of grooms, grouped by launch class (the class with
+// the most expected work first; within a class, grooms with more iterations
+// first).  Linear in the number of tiles (only the grooms are sorted): the
+// serving loop plans a batch per launch.
 ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::vector<FtlTile>& tiles,
                           std::vector<ClassRun>& runs) {
   tiles.clear();
@@ -70,7 +68,6 @@ ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::ve
   std::vector<GroomPlan> plan;
   plan.reserve(n);
   std::vector<double> work(FTL_NCLASSES, 0.0);
-  std::vector<int> cls_cost(FTL_NCLASSES, -1);
   size_t total_tiles = 0;
   for (int i = 0; i < n; ++i) {
     ftl_groom* g = grooms[i];
@@ -85,7 +82,6 @@ ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::ve
     const double ns_tile = class_ns_per_cp(ctx, cls, g->params.iterations) * (double)FTL_WAVE * k.m;
     plan.push_back(GroomPlan{i, cls, cost, nt, ns_tile});
     work[cls] += ns_tile * nt;
-    cls_cost[cls] = std::max(cls_cost[cls], cost);
     total_tiles += (size_t)nt;
   }
   if (total_tiles > (size_t)INT32_MAX) return ftl_fail(FTL_ERR_UNSUPPORTED, "too many tiles in one batch");
@@ -95,19 +91,18 @@ ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::ve
     if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
       return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "a groom appears twice in the batch");
   }
+  // Classes with the most expected work first (they start first and run
+  // longest); within a class, the costliest grooms first.
   std::stable_sort(plan.begin(), plan.end(), [&](const GroomPlan& a, const GroomPlan& b) {
     if (a.cls != b.cls) {
-      if (cls_cost[a.cls] != cls_cost[b.cls]) return cls_cost[a.cls] > cls_cost[b.cls];
+      if (work[a.cls] != work[b.cls]) return work[a.cls] > work[b.cls];
       return a.cls < b.cls;
     }
     return a.cost > b.cost;
   });
   tiles.reserve(total_tiles);
-  // The classes run side by side.  Each gets a share of the device's wave
-  // slots in proportion to its expected time on the whole device, so that
-  // all of them drain at about the same time.
-  double total = 0.0;
-  for (double w : work) total += w;
+  // One workgroup per tile: the classes' kernels run side by side and the
+  // hardware dispatcher hands free wave slots to whichever has tiles left.
   for (size_t i = 0; i < plan.size();) {
     const int cls = plan[i].cls;
     const int first = (int)tiles.size();
@@ -117,33 +112,26 @@ ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::ve
       for (int t = 0; t < gp.ntiles; ++t) tiles.push_back(FtlTile{gp.idx, t * spt, cls, gp.cost});
     }
     const int count = (int)tiles.size() - first;
-    int per_cu = ftl_persist_blocks_per_cu(cls);
-    if (per_cu <= 0) per_cu = 4;
-    const double share = total > 0.0 ? work[cls] / total : 1.0;
-    const long slots = (long)((double)ctx->ncu * per_cu * share + 0.999);
-    runs.push_back(ClassRun{cls, first, count, (int)std::max(1L, std::min<long>(slots, count))});
+    runs.push_back(ClassRun{cls, first, count, count});
   }
   if (runs.size() > 64) return ftl_fail(FTL_ERR_UNSUPPORTED, "too many launch classes");
   return FTL_OK;
 }
 
-// Enqueues the counter reset and one persistent kernel per class on s.  With
-// side streams (graph capture) the classes after the first are forked onto
-// them and joined back, so they become parallel graph branches.
+// Enqueues one tile-list kernel per class on s.  With side streams (graph
+// capture, serving slots) the classes after the first are forked onto them
+// and joined back, so they run side by side (parallel graph branches).
 hipError_t ftl_enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_steps, const FtlTile* d_tiles,
-                               unsigned* d_counter, hipStream_t s, const std::vector<hipStream_t>* side,
-                               const std::vector<hipEvent_t>* side_ev, hipEvent_t fork) {
-  hipError_t e = ftl_reset_counters(d_counter, (int)runs.size(), s);
-  if (e != hipSuccess) return e;
+                            hipStream_t s, const std::vector<hipStream_t>* side, const std::vector<hipEvent_t>* side_ev,
+                            hipEvent_t fork) {
+  hipError_t e;
   const bool par = side && runs.size() > 1;
   if (par && (e = hipEventRecord(fork, s)) != hipSuccess) return e;
   for (size_t r = 0; r < runs.size(); ++r) {
     hipStream_t rs = (par && r > 0) ? (*side)[r - 1] : s;
     if (par && r > 0 && (e = hipStreamWaitEvent(rs, fork, 0)) != hipSuccess) return e;
     const ClassRun& cr = runs[r];
-    if ((e = ftl_launch_persist(cr.cls, d_steps, d_tiles + cr.first, cr.count, d_counter + r, cr.grid, rs)) !=
-        hipSuccess)
-      return e;
+    if ((e = ftl_launch_tiles(cr.cls, d_steps, d_tiles + cr.first, cr.count, cr.grid, rs)) != hipSuccess) return e;
     if (par && r > 0) {
       if ((e = hipEventRecord((*side_ev)[r - 1], rs)) != hipSuccess) return e;
       if ((e = hipStreamWaitEvent(s, (*side_ev)[r - 1], 0)) != hipSuccess) return e;
@@ -181,15 +169,12 @@ extern "C" ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int
     if ((st = ftl_dev_alloc(ctx, &ctx->scratch_tiles, tb, "batch tiles")) != FTL_OK) return st;
     ctx->scratch_tiles_bytes = tb;
   }
-  if (!ctx->scratch_counter &&
-      (st = ftl_dev_alloc(ctx, (void**)&ctx->scratch_counter, kCounterBytes, "batch counters")) != FTL_OK)
-    return st;
   if ((e = hipMemcpy(ctx->scratch_steps, steps.data(), sb, hipMemcpyHostToDevice)) != hipSuccess)
     return ftl_hip_fail(e, "hipMemcpy batch descriptors");
   if ((e = hipMemcpy(ctx->scratch_tiles, tiles.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
     return ftl_hip_fail(e, "hipMemcpy batch tiles");
   if ((e = ftl_enqueue_runs(runs, (const FtlStep*)ctx->scratch_steps, (const FtlTile*)ctx->scratch_tiles,
-                        ctx->scratch_counter, (hipStream_t)stream, nullptr, nullptr, nullptr)) != hipSuccess)
+                            (hipStream_t)stream, nullptr, nullptr, nullptr)) != hipSuccess)
     return ftl_hip_fail(e, "batched step launch");
   return FTL_OK;
 }
@@ -204,7 +189,6 @@ static void batch_release(ftl_batch* b) {
     if (b->done[k]) (void)hipEventDestroy(b->done[k]);
     if (b->h_steps[k]) (void)hipHostFree(b->h_steps[k]);
     ftl_dev_free(ctx, b->d_steps[k], b->grooms.size() * sizeof(FtlStep));
-    ftl_dev_free(ctx, b->d_counter[k], kCounterBytes);
   }
   ftl_dev_free(ctx, b->d_tiles, (size_t)b->ntiles * sizeof(FtlTile));
   for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
@@ -253,8 +237,6 @@ extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, i
   }
   for (int k = 0; k < 2; ++k) {
     if ((st = ftl_dev_alloc(ctx, (void**)&b->d_steps[k], sb, "batch descriptors")) != FTL_OK) return bail(st);
-    if ((st = ftl_dev_alloc(ctx, (void**)&b->d_counter[k], kCounterBytes, "batch counters")) != FTL_OK)
-      return bail(st);
     if ((e = hipHostMalloc((void**)&b->h_steps[k], sb, hipHostMallocDefault)) != hipSuccess)
       return bail(ftl_hip_fail(e, "hipHostMalloc"));
     std::memset(b->h_steps[k], 0, sb);
@@ -263,12 +245,12 @@ extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, i
     if ((e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return bail(ftl_hip_fail(e, "hipStreamBeginCapture"));
     hipError_t e1 = hipMemcpyAsync(b->d_steps[k], b->h_steps[k], sb, hipMemcpyHostToDevice, b->cap);
-    hipError_t e2 = e1 == hipSuccess ? ftl_enqueue_runs(b->runs, b->d_steps[k], b->d_tiles, b->d_counter[k], b->cap,
-                                                    &b->side, &b->side_ev, b->fork)
-                                     : hipSuccess;
+    hipError_t e2 = e1 == hipSuccess
+                        ? ftl_enqueue_runs(b->runs, b->d_steps[k], b->d_tiles, b->cap, &b->side, &b->side_ev, b->fork)
+                        : hipSuccess;
     hipError_t e3 = hipStreamEndCapture(b->cap, &b->graph[k]);
     if (e1 != hipSuccess) return bail(ftl_hip_fail(e1, "capture: descriptor upload"));
-    if (e2 != hipSuccess) return bail(ftl_hip_fail(e2, "capture: persistent kernels"));
+    if (e2 != hipSuccess) return bail(ftl_hip_fail(e2, "capture: tile kernels"));
     if (e3 != hipSuccess) return bail(ftl_hip_fail(e3, "hipStreamEndCapture"));
     if ((e = hipGraphInstantiate(&b->exec[k], b->graph[k], nullptr, nullptr, 0)) != hipSuccess)
       return bail(ftl_hip_fail(e, "hipGraphInstantiate"));

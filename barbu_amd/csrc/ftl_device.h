@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "ftl.h"
 
 #define FTL_WAVE 64
 #define FTL_NCOL 4
@@ -127,37 +128,34 @@ __device__ __forceinline__ float lerp_c(float a, float b, float t) { return __bu
 // Wind acceleration at x: trilinear value noise over a periodic lattice with
 // smoothstep weights (the force hook of CalculateForces, cs_simulation.glsl:70).
 __device__ __forceinline__ float3 wind_at(const FtlStep& A, float3 x) {
-  const int G = A.wind_grid, msk = G - 1;
+  // The lattice side is the constant FTL_WIND_GRID (a power of two): corner
+  // indices are masks and shifts, and every offset fits 32 bits, so the
+  // gathers address the table from its scalar base.
+  constexpr unsigned G = FTL_WIND_GRID, msk = FTL_WIND_GRID - 1;
   const float u0 = __builtin_fmaf(x.x, A.wind_freq, A.wind_off[0]);
   const float u1 = __builtin_fmaf(x.y, A.wind_freq, A.wind_off[1]);
   const float u2 = __builtin_fmaf(x.z, A.wind_freq, A.wind_off[2]);
   const float l0 = floorf(u0), l1 = floorf(u1), l2 = floorf(u2);
   const float f0 = u0 - l0, f1 = u1 - l1, f2 = u2 - l2;
-  const int i0 = (int)l0, i1 = (int)l1, i2 = (int)l2;
-  const int x0 = i0 & msk, x1 = (i0 + 1) & msk;
-  const int y0 = i1 & msk, y1 = (i1 + 1) & msk;
-  const int z0 = i2 & msk, z1 = (i2 + 1) & msk;
+  const unsigned i0 = (unsigned)(int)l0, i1 = (unsigned)(int)l1, i2 = (unsigned)(int)l2;
+  const unsigned x0 = i0 & msk, x1 = (i0 + 1u) & msk;
+  const unsigned y0 = i1 & msk, y1 = (i1 + 1u) & msk;
+  const unsigned z0 = i2 & msk, z1 = (i2 + 1u) & msk;
   const float s0 = (f0 * f0) * __builtin_fmaf(-2.0f, f0, 3.0f);
   const float s1 = (f1 * f1) * __builtin_fmaf(-2.0f, f1, 3.0f);
   const float s2 = (f2 * f2) * __builtin_fmaf(-2.0f, f2, 3.0f);
   const float4* T = A.wind;
-  // Four x-edges of the cell, gathered one edge (two corners) at a time so the
-  // register peak stays small: c_yz = lerp(v[z][y][x0], v[z][y][x1], s0).
-  const int r00 = (z0 * G + y0) * G, r10 = (z0 * G + y1) * G, r01 = (z1 * G + y0) * G, r11 = (z1 * G + y1) * G;
-  float4 a0 = T[r00 + x0], a1 = T[r00 + x1];
-  const float3 c00 = f3(lerp_c(a0.x, a1.x, s0), lerp_c(a0.y, a1.y, s0), lerp_c(a0.z, a1.z, s0));
-  __builtin_amdgcn_sched_barrier(0);
-  a0 = T[r10 + x0];
-  a1 = T[r10 + x1];
-  const float3 c10 = f3(lerp_c(a0.x, a1.x, s0), lerp_c(a0.y, a1.y, s0), lerp_c(a0.z, a1.z, s0));
-  __builtin_amdgcn_sched_barrier(0);
-  a0 = T[r01 + x0];
-  a1 = T[r01 + x1];
-  const float3 c01 = f3(lerp_c(a0.x, a1.x, s0), lerp_c(a0.y, a1.y, s0), lerp_c(a0.z, a1.z, s0));
-  __builtin_amdgcn_sched_barrier(0);
-  a0 = T[r11 + x0];
-  a1 = T[r11 + x1];
-  const float3 c11 = f3(lerp_c(a0.x, a1.x, s0), lerp_c(a0.y, a1.y, s0), lerp_c(a0.z, a1.z, s0));
+  // The eight corners are gathered together: one memory round trip per point.
+  const unsigned r00 = (z0 * G + y0) * G, r10 = (z0 * G + y1) * G, r01 = (z1 * G + y0) * G,
+                 r11 = (z1 * G + y1) * G;
+  const float4 a00 = T[r00 + x0], b00 = T[r00 + x1];
+  const float4 a10 = T[r10 + x0], b10 = T[r10 + x1];
+  const float4 a01 = T[r01 + x0], b01 = T[r01 + x1];
+  const float4 a11 = T[r11 + x0], b11 = T[r11 + x1];
+  const float3 c00 = f3(lerp_c(a00.x, b00.x, s0), lerp_c(a00.y, b00.y, s0), lerp_c(a00.z, b00.z, s0));
+  const float3 c10 = f3(lerp_c(a10.x, b10.x, s0), lerp_c(a10.y, b10.y, s0), lerp_c(a10.z, b10.z, s0));
+  const float3 c01 = f3(lerp_c(a01.x, b01.x, s0), lerp_c(a01.y, b01.y, s0), lerp_c(a01.z, b01.z, s0));
+  const float3 c11 = f3(lerp_c(a11.x, b11.x, s0), lerp_c(a11.y, b11.y, s0), lerp_c(a11.z, b11.z, s0));
   float3 r;
   r.x = lerp_c(lerp_c(c00.x, c10.x, s1), lerp_c(c01.x, c11.x, s1), s2);
   r.y = lerp_c(lerp_c(c00.y, c10.y, s1), lerp_c(c01.y, c11.y, s1), s2);

@@ -24,7 +24,6 @@ struct ftl_ctx {
   size_t scratch_steps_bytes = 0;
   void* scratch_tiles = nullptr;
   size_t scratch_tiles_bytes = 0;
-  unsigned* scratch_counter = nullptr;
 };
 
 struct ftl_skin {
@@ -78,13 +77,12 @@ int ftl_class_for(ftl_ctx* ctx, int npoints, int lanes, int mmax);
 struct ClassRun {
   int cls, first, count, grid;
 };
-static const int kCounterBytes = 256;  // tile counters: up to 64 classes
 // Cuts the grooms into tiles grouped by class and sizes the class grids.
 ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::vector<FtlTile>& tiles,
                           std::vector<ClassRun>& runs);
-// Enqueues the counter reset and one persistent kernel per class on s; with
-// side streams (and their events) the classes after the first run on them,
-// forked from and joined back to s.
+// Enqueues one tile-list kernel per class on s; with side streams (and their
+// events) the classes after the first run on them, forked from and joined
+// back to s.
 hipError_t ftl_enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_steps, const FtlTile* d_tiles,
-                            unsigned* d_counter, hipStream_t s, const std::vector<hipStream_t>* side,
-                            const std::vector<hipEvent_t>* side_ev, hipEvent_t fork);
+                            hipStream_t s, const std::vector<hipStream_t>* side, const std::vector<hipEvent_t>* side_ev,
+                            hipEvent_t fork);

@@ -1,4 +1,4 @@
-// Host-side entry points of the kernels (ftl_kernels.hip), used by the C ABI.
+// Host-side entry points of the kernels (ftl_plain.hip, ftl_misc.hip), used by the C ABI.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,13 +12,12 @@ int ftl_class_g(int cls);
 int ftl_class_guarded(int cls);
 
 hipError_t ftl_launch_plain(int cls, const FtlStep& A, hipStream_t stream);
-// Zeroes n tile counters (at most 64) with a kernel (no memset node in graphs).
-hipError_t ftl_reset_counters(unsigned* counters, int n, hipStream_t stream);
-// Persistent kernel of one class over tiles[0..ntiles) (all of that class).
-hipError_t ftl_launch_persist(int cls, const FtlStep* grooms, const FtlTile* tiles, int ntiles, unsigned* counter,
-                              int grid, hipStream_t stream);
-// Resident persistent workgroups per CU for a class.
-int ftl_persist_blocks_per_cu(int cls);
+// Tile-list kernel of one class over tiles[0..ntiles) (all of that class):
+// workgroup b steps tiles b, b + grid, ...
+hipError_t ftl_launch_tiles(int cls, const FtlStep* grooms, const FtlTile* tiles, int ntiles, int grid,
+                            hipStream_t stream);
+// Resident tile-list workgroups per CU for a class.
+int ftl_tiles_blocks_per_cu(int cls);
 hipError_t ftl_launch_skin(int nverts, const float4* dq, const int4* joints, const float4* weights,
                            const float4* rest, float4* out, hipStream_t stream);
 hipError_t ftl_launch_copy(const void* src, void* dst, size_t n16, int ncu, hipStream_t stream);

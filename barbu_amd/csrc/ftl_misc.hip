@@ -1,15 +1,10 @@
-// Small kernels and host dispatch: tile-counter reset, dual-quaternion
-// skinning, and the class table -> per-class launcher map.
+// Small kernels and host dispatch: dual-quaternion skinning, the copy
+// reference, and the class table -> per-class launcher map.
 #include "ftl_classes.h"
 #include "ftl_device.h"
 #include "ftl_launch.h"
 
 using namespace ftl_dev;
-
-__global__ void ftl_reset_counters_kernel(unsigned* counters, int n) {
-  const int i = threadIdx.x;
-  if (blockIdx.x == 0 && i < n) counters[i] = 0u;
-}
 
 // ---------------------------------------------------------------------------
 // Dual-quaternion blended skinning of mesh vertices
@@ -83,16 +78,16 @@ __global__ __launch_bounds__(256) void ftl_copy_kernel(const float4* __restrict_
 
 #define FTL_DECL(id, M, G, GU, W, CH)                                                                    \
   hipError_t ftl_launch_plain_##id(const FtlStep& A, hipStream_t stream);                           \
-  hipError_t ftl_launch_persist_##id(const FtlStep*, const FtlTile*, int, unsigned*, int, hipStream_t); \
-  int ftl_persist_occ_##id();
+  hipError_t ftl_launch_tiles_##id(const FtlStep*, const FtlTile*, int, int, hipStream_t);          \
+  int ftl_tiles_occ_##id();
 FTL_CLASSES(FTL_DECL)
 #undef FTL_DECL
 
-int ftl_persist_blocks_per_cu(int cls) {
+int ftl_tiles_blocks_per_cu(int cls) {
   switch (cls) {
 #define FTL_OC(id, M, G, GU, W, CH) \
   case id:                   \
-    return ftl_persist_occ_##id();
+    return ftl_tiles_occ_##id();
     FTL_CLASSES(FTL_OC)
 #undef FTL_OC
   }
@@ -115,18 +110,12 @@ hipError_t ftl_launch_plain(int cls, const FtlStep& A, hipStream_t stream) {
   return hipErrorInvalidValue;
 }
 
-hipError_t ftl_reset_counters(unsigned* counters, int n, hipStream_t stream) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(ftl_reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters, n);
-  return hipGetLastError();
-}
-
-hipError_t ftl_launch_persist(int cls, const FtlStep* grooms, const FtlTile* tiles, int ntiles, unsigned* counter,
-                              int grid, hipStream_t stream) {
+hipError_t ftl_launch_tiles(int cls, const FtlStep* grooms, const FtlTile* tiles, int ntiles, int grid,
+                            hipStream_t stream) {
   switch (cls) {
 #define FTL_LB(id, M, G, GU, W, CH) \
   case id:                   \
-    return ftl_launch_persist_##id(grooms, tiles, ntiles, counter, grid, stream);
+    return ftl_launch_tiles_##id(grooms, tiles, ntiles, grid, stream);
     FTL_CLASSES(FTL_LB)
 #undef FTL_LB
   }

@@ -10,7 +10,7 @@
 //      creature, up to the per-batch traffic cap), applies each request's
 //      inputs to its groom and launches one batched persistent step on the
 //      slot's streams (tile plan -> pinned descriptors -> async copies ->
-//      one persistent kernel per launch class, classes side by side).
+//      one tile-list kernel per launch class, classes side by side).
 // Slots overlap on the device, so requests that arrive while a batch runs
 // start as soon as a slot frees, not when the device drains.
 #include <hip/hip_runtime.h>
@@ -71,7 +71,6 @@ struct Slot {
   FtlTile* h_tiles = nullptr;
   FtlTile* d_tiles = nullptr;
   size_t tiles_cap = 0;
-  unsigned* d_counter = nullptr;
   std::vector<Req> reqs;
   bool busy = false;
 };
@@ -241,8 +240,7 @@ ftl_status launch_batch(ftl_server* s, Slot& sl) {
   if ((e = hipMemcpyAsync(sl.d_tiles, sl.h_tiles, tiles.size() * sizeof(FtlTile), hipMemcpyHostToDevice, sl.s)) !=
       hipSuccess)
     return ftl_hip_fail(e, "server tile upload");
-  if ((e = ftl_enqueue_runs(runs, sl.d_steps, sl.d_tiles, sl.d_counter, sl.s, &sl.side, &sl.side_ev, sl.fork)) !=
-      hipSuccess)
+  if ((e = ftl_enqueue_runs(runs, sl.d_steps, sl.d_tiles, sl.s, &sl.side, &sl.side_ev, sl.fork)) != hipSuccess)
     return ftl_hip_fail(e, "server batch launch");
   if ((e = hipEventRecord(sl.done, sl.s)) != hipSuccess) return ftl_hip_fail(e, "hipEventRecord");
   return FTL_OK;
@@ -401,7 +399,6 @@ extern "C" ftl_status ftl_server_create(ftl_ctx* ctx, const ftl_server_config* c
         (e = hipEventCreateWithFlags(&sl.fork, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess)
       st = ftl_hip_fail(e, "server streams");
-    if (st == FTL_OK) st = ftl_dev_alloc(ctx, (void**)&sl.d_counter, kCounterBytes, "server counters");
     if (st != FTL_OK) {
       ftl_server_destroy(s);
       return st;
@@ -445,7 +442,6 @@ extern "C" void ftl_server_destroy(ftl_server* s) {
     if (sl.h_tiles) (void)hipHostFree(sl.h_tiles);
     ftl_dev_free(s->ctx, sl.d_steps, sl.steps_cap * sizeof(FtlStep));
     ftl_dev_free(s->ctx, sl.d_tiles, sl.tiles_cap * sizeof(FtlTile));
-    ftl_dev_free(s->ctx, sl.d_counter, kCounterBytes);
   }
   if (started) {
     std::lock_guard<std::mutex> lk(s->ctx->mu);

@@ -204,20 +204,22 @@ ftl_status ftl_groom_bind_skin(ftl_groom* g, ftl_skin* s, const int* tri, const 
 /* -- batched steps --------------------------------------------------------- */
 
 /* Steps n grooms of one context as one batch: the grooms are cut into 64-lane
-   tiles, and one persistent kernel per launch class present pulls its tiles
-   (the classes run side by side).  dt: one value per groom.  For repeated frames over the same
+   tiles, and one tile-list kernel per launch class present steps them (the
+   classes run side by side).  dt: one value per groom.  For repeated frames over the same
    grooms, ftl_batch_create / ftl_batch_launch avoid re-planning. */
 ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int n, const float* dt,
                           void* stream);
 
 /* A fixed set of grooms stepped as one frame, captured once into a HIP graph
-   (parameter upload + persistent kernel) and replayed by ftl_batch_launch. */
+   (parameter upload + one tile-list kernel per launch class) and replayed by
+   ftl_batch_launch. */
 ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, int n, ftl_batch** out);
 void ftl_batch_destroy(ftl_batch* b);
 /* One frame: per-groom dt (n values).  Colliders, transforms, params and time
    are read from the grooms at call time.  Asynchronous on `stream`. */
 ftl_status ftl_batch_launch(ftl_batch* b, const float* dt, void* stream);
-/* Tiles (64-lane work units) in the batch and the persistent grid size. */
+/* Tiles (64-lane work units) in the batch and the total number of
+   workgroups its kernels launch. */
 ftl_status ftl_batch_info(ftl_batch* b, int* ntiles, int* grid);
 
 /* -- helpers -------------------------------------------------------------- */

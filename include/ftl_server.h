@@ -11,8 +11,8 @@
  *     device) only if the resident bytes stay within the budget; otherwise it
  *     waits, and is admitted, in order of arrival, when a removal frees memory.
  *   - Continuous batching: a dispatcher thread collects the pending step
- *     requests of resident creatures and launches them as one batched,
- *     persistent step (the kernels of ftl_batch_step), up to a per-batch cap
+ *     requests of resident creatures and launches them as one batched
+ *     step (the kernels of ftl_batch_step), up to a per-batch cap
  *     on HBM traffic (64 B per control point).  Requests that arrive while
  *     batches run join the next batch as soon as one of the in-flight slots
  *     frees; up to `max_inflight` batches overlap on the device.
