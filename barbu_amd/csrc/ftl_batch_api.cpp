@@ -73,7 +73,7 @@ ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::ve
     ftl_groom* g = grooms[i];
     if (!g) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom %d is NULL", i);
     if (g->ctx != ctx) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom %d belongs to another context", i);
-    const int cls = ftl_class_for(ctx, g->npoints, g->lanes, 32);
+    const int cls = ftl_class_for(ctx, g->npoints, g->lanes, FTL_MAX_POINTS);
     if (cls < 0) return ftl_fail(FTL_ERR_UNSUPPORTED, "no batched launch class for npoints=%d", g->npoints);
     const FtlClassInfo& k = kFtlClasses[cls];
     const int spt = FTL_WAVE / k.g;

@@ -182,7 +182,10 @@ __device__ __forceinline__ void sweep(const FtlStep& A, const Spheres& S, float3
         last_pre = pn;
       }
     } else {
-      if (j >= 2) {
+      // Point j-1 is collided once, after the point it leads was swept.  The
+      // strand's last point is collided right after its own sweep (below);
+      // with GUARD the slots past it are not collided at all.
+      if (j >= 2 && (!GUARD || j - 1 < nl - 1)) {
         float3 dummy;
         collide<false>(S, p[j - 1], dummy, cm);
       }

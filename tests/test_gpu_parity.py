@@ -115,6 +115,16 @@ def test_guarded_point_counts(ctx, npoints):
     run_compare(ctx, pos, vel, params, [(0, 0, 0, 1)], 3, label="N=%d" % npoints)
 
 
+@pytest.mark.parametrize("npoints", [3, 5, 8, 20, 32, 48])
+def test_tips_inside_a_collider(ctx, npoints):
+    """A sphere that contains the strand tips: every constraint iteration
+    pushes the last points, which must be collided exactly once per
+    iteration (a second push moves a point on the surface by rounding)."""
+    pos, vel, _ = ba.sphere_groom(400, npoints, radius=1.0, length=0.25, seed=50 + npoints)
+    params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0)
+    run_compare(ctx, pos, vel, params, [(0, 0, 0, 1.15)], 3, label="tips N=%d" % npoints)
+
+
 def test_dftl_mode_with_clamp(ctx):
     pos, vel, _ = ba.sphere_groom(700, 16, radius=1.0, length=0.2, seed=11)
     vel[:, 1:, :3] = np.random.default_rng(1).normal(scale=3.0, size=vel[:, 1:, :3].shape)
@@ -182,7 +192,9 @@ def test_nonfinite_flag(ctx):
 def _mixed_grooms(ctx, seed=0):
     rng = np.random.default_rng(seed)
     grooms, states = [], []
-    for i, (S, N) in enumerate([(1000, 8), (130, 16), (777, 32), (64, 64), (2049, 16), (5, 32), (900, 64)]):
+    # exact classes and ragged point counts (guarded classes)
+    for i, (S, N) in enumerate([(1000, 8), (130, 16), (777, 32), (64, 64), (2049, 16), (5, 32), (900, 64),
+                                (300, 5), (200, 20), (100, 48), (50, 2)]):
         pos, vel, roots = ba.sphere_groom(S, N, center=tuple(rng.normal(size=3) * 2), length=float(rng.uniform(0.05, 0.4)),
                                           seed=100 + i)
         g = ba.Groom(ctx, pos, vel, rest_roots=roots)
@@ -218,7 +230,7 @@ def test_batched_step_matches_oracle(ctx):
         models = [_rigid(k, i) for i in range(len(grooms))]
         for g, m in zip(grooms, models):
             g.set_root_transform(m)
-        dts = [np.float32(1 / 60), np.float32(1 / 90)] * 4
+        dts = [np.float32(1 / 60), np.float32(1 / 90)] * 8
         dts = dts[: len(grooms)]
         ba.batch_step(ctx, grooms, dts)
         _oracle_frame(states, dts, models)
