@@ -294,7 +294,7 @@ extern "C" void ftl_groom_destroy(ftl_groom* g) {
   ftl_dev_free(ctx, g->d_status, 256);
   ftl_dev_free(ctx, g->d_roots, (size_t)g->nstrands * sizeof(float4));
   ftl_dev_free(ctx, g->d_bind, (size_t)g->nstrands * sizeof(int4));
-  ftl_dev_free(ctx, g->d_wind, (size_t)FTL_WIND_GRID * FTL_WIND_GRID * FTL_WIND_GRID * sizeof(float4));
+  ftl_dev_free(ctx, g->d_wind, (size_t)FTL_WIND_GRID * FTL_WIND_GRID * FTL_WIND_GRID * 2 * sizeof(float4));
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->live_objects--;
@@ -384,20 +384,37 @@ extern "C" ftl_status ftl_groom_set_lanes(ftl_groom* g, int lanes) {
 }
 
 // Uploads the wind lattice when the wind parameters changed.
+// Uploads the wind lattice when the wind parameters changed, in the device
+// layout of the kernels: per node (v.x, v.y, v.z, d.x), (d.y, d.z, 0, 0) with
+// d = v(x+1) - v(x) along the periodic x axis.
 static ftl_status ensure_wind(ftl_groom* g) {
   const ftl_params& p = g->params;
   if (p.wind_amplitude == 0.0f) return FTL_OK;
   if (g->d_wind && g->wind_seed == p.wind_seed && g->wind_amp == p.wind_amplitude) return FTL_OK;
-  const size_t cells = (size_t)FTL_WIND_GRID * FTL_WIND_GRID * FTL_WIND_GRID;
+  const int G = FTL_WIND_GRID;
+  const size_t cells = (size_t)G * G * G;
+  const size_t bytes = cells * 2 * sizeof(float4);
   if (!g->d_wind) {
-    ftl_status st = ftl_dev_alloc(g->ctx, (void**)&g->d_wind, cells * sizeof(float4), "wind lattice");
+    ftl_status st = ftl_dev_alloc(g->ctx, (void**)&g->d_wind, bytes, "wind lattice");
     if (st != FTL_OK) return st;
   }
-  std::vector<float> h(cells * 4);
+  std::vector<float> h(cells * 4), t(cells * 8, 0.0f);
   ftl_wind_lattice(&p, h.data());
+  for (size_t n = 0; n < cells; ++n) {
+    const size_t x = n % G, row = n - x;
+    const float* a = &h[4 * n];
+    const float* b = &h[4 * (row + (x + 1) % G)];
+    float* o = &t[8 * n];
+    o[0] = a[0];
+    o[1] = a[1];
+    o[2] = a[2];
+    o[3] = b[0] - a[0];
+    o[4] = b[1] - a[1];
+    o[5] = b[2] - a[2];
+  }
   hipError_t e;
   if ((e = hipDeviceSynchronize()) != hipSuccess) return ftl_hip_fail(e, "hipDeviceSynchronize");
-  if ((e = hipMemcpy(g->d_wind, h.data(), cells * sizeof(float4), hipMemcpyHostToDevice)) != hipSuccess)
+  if ((e = hipMemcpy(g->d_wind, t.data(), bytes, hipMemcpyHostToDevice)) != hipSuccess)
     return ftl_hip_fail(e, "hipMemcpy wind lattice");
   g->wind_seed = p.wind_seed;
   g->wind_amp = p.wind_amplitude;

@@ -26,7 +26,7 @@ struct FtlStep {
   const int4* bind;             // FTL_ROOT_SKINNED: {triangle, w1 bits, w2 bits, 0}
   const int4* tris;             // FTL_ROOT_SKINNED: {a, b, c, 0} per triangle
   const float4* skinned;        // FTL_ROOT_SKINNED: skinned vertices
-  const float4* wind;           // wind lattice, float4 [grid^3], or null
+  const float4* wind;           // wind lattice, 2 float4 per node (layout at wind_at), or null
   unsigned* status;             // bit 0: a strand tip became non-finite
   int nstrands, npoints, iterations, vmode;
   int rmode, ncol, model_ident, wind_grid;
@@ -127,6 +127,16 @@ __device__ __forceinline__ float lerp_c(float a, float b, float t) { return __bu
 
 // Wind acceleration at x: trilinear value noise over a periodic lattice with
 // smoothstep weights (the force hook of CalculateForces, cs_simulation.glsl:70).
+// Device layout of the lattice: two float4 per node n = (z*G + y)*G + x,
+//   T[2n] = (v.x, v.y, v.z, d.x),  T[2n+1] = (d.y, d.z, 0, 0),
+// with d = v(x+1, y, z) - v(x, y, z) rounded once when the table is built, so
+// an x-edge is fma(s0, d, v): the same value as lerp(v, v(x+1), s0) computed
+// with the subtraction at run time.
+__device__ __forceinline__ float3 wind_edge(const float4* __restrict__ T, unsigned n, float s0) {
+  const float4 e0 = T[2u * n], e1 = T[2u * n + 1u];
+  return f3(__builtin_fmaf(s0, e0.w, e0.x), __builtin_fmaf(s0, e1.x, e0.y), __builtin_fmaf(s0, e1.y, e0.z));
+}
+
 __device__ __forceinline__ float3 wind_at(const FtlStep& A, float3 x) {
   // The lattice side is the constant FTL_WIND_GRID (a power of two): corner
   // indices are masks and shifts, and every offset fits 32 bits, so the
@@ -138,24 +148,18 @@ __device__ __forceinline__ float3 wind_at(const FtlStep& A, float3 x) {
   const float l0 = floorf(u0), l1 = floorf(u1), l2 = floorf(u2);
   const float f0 = u0 - l0, f1 = u1 - l1, f2 = u2 - l2;
   const unsigned i0 = (unsigned)(int)l0, i1 = (unsigned)(int)l1, i2 = (unsigned)(int)l2;
-  const unsigned x0 = i0 & msk, x1 = (i0 + 1u) & msk;
+  const unsigned x0 = i0 & msk;
   const unsigned y0 = i1 & msk, y1 = (i1 + 1u) & msk;
   const unsigned z0 = i2 & msk, z1 = (i2 + 1u) & msk;
   const float s0 = (f0 * f0) * __builtin_fmaf(-2.0f, f0, 3.0f);
   const float s1 = (f1 * f1) * __builtin_fmaf(-2.0f, f1, 3.0f);
   const float s2 = (f2 * f2) * __builtin_fmaf(-2.0f, f2, 3.0f);
-  const float4* T = A.wind;
-  // The eight corners are gathered together: one memory round trip per point.
-  const unsigned r00 = (z0 * G + y0) * G, r10 = (z0 * G + y1) * G, r01 = (z1 * G + y0) * G,
-                 r11 = (z1 * G + y1) * G;
-  const float4 a00 = T[r00 + x0], b00 = T[r00 + x1];
-  const float4 a10 = T[r10 + x0], b10 = T[r10 + x1];
-  const float4 a01 = T[r01 + x0], b01 = T[r01 + x1];
-  const float4 a11 = T[r11 + x0], b11 = T[r11 + x1];
-  const float3 c00 = f3(lerp_c(a00.x, b00.x, s0), lerp_c(a00.y, b00.y, s0), lerp_c(a00.z, b00.z, s0));
-  const float3 c10 = f3(lerp_c(a10.x, b10.x, s0), lerp_c(a10.y, b10.y, s0), lerp_c(a10.z, b10.z, s0));
-  const float3 c01 = f3(lerp_c(a01.x, b01.x, s0), lerp_c(a01.y, b01.y, s0), lerp_c(a01.z, b01.z, s0));
-  const float3 c11 = f3(lerp_c(a11.x, b11.x, s0), lerp_c(a11.y, b11.y, s0), lerp_c(a11.z, b11.z, s0));
+  const float4* __restrict__ T = A.wind;
+  // The four x-edges of the cell, gathered together (one round trip).
+  const float3 c00 = wind_edge(T, (z0 * G + y0) * G + x0, s0);
+  const float3 c10 = wind_edge(T, (z0 * G + y1) * G + x0, s0);
+  const float3 c01 = wind_edge(T, (z1 * G + y0) * G + x0, s0);
+  const float3 c11 = wind_edge(T, (z1 * G + y1) * G + x0, s0);
   float3 r;
   r.x = lerp_c(lerp_c(c00.x, c10.x, s1), lerp_c(c01.x, c11.x, s1), s2);
   r.y = lerp_c(lerp_c(c00.y, c10.y, s1), lerp_c(c01.y, c11.y, s1), s2);

@@ -114,7 +114,7 @@ namespace {
 uint64_t creature_bytes(int nstrands, int npoints, bool roots, const ftl_params& p) {
   uint64_t b = (uint64_t)nstrands * npoints * 32u;
   if (roots) b += (uint64_t)nstrands * 16u;
-  if (p.wind_amplitude != 0.0f) b += (uint64_t)FTL_WIND_GRID * FTL_WIND_GRID * FTL_WIND_GRID * 16u;
+  if (p.wind_amplitude != 0.0f) b += (uint64_t)FTL_WIND_GRID * FTL_WIND_GRID * FTL_WIND_GRID * 32u;
   return b + 256u;
 }
 
