@@ -58,6 +58,13 @@ struct Tile {
   static constexpr int SPT = FTL_WAVE / G;   // strands per tile
 };
 
+// Pipelined forms stage their final velocities through LDS (whole-line
+// stores) when a lane holds at most two chunks; with more, the flushes inside
+// the unrolled sweep cost more registers than the partial-line stores they
+// replace, and each lane stores its velocities directly.
+template <int M, int G>
+constexpr bool kStagedVel = (M / Tile<M, G, false>::C) <= 2;
+
 // LDS slot of (owner lane r, point j of the chunk): XOR swizzle, conflict-free
 // for both the per-lane and the coalesced access order.
 template <int C>
@@ -112,11 +119,13 @@ __device__ __forceinline__ float3 root_source(const FtlStep& A, int s, float4 x)
   return f3(x.x, x.y, x.z);
 }
 
-// Writes chunk c of the tile (C points per lane, staged in LDS by every lane)
-// to dst in the coalesced order of the load.  Called by the whole wave.
+// Writes chunk c of the tile (C points per lane, staged in LDS) to dst in
+// the coalesced order of the load.  Called by the whole wave.  gsel >= 0
+// writes only the points owned by lanes with g == gsel (the lanes of a
+// strand finish at different time steps in the pipelined forms).
 template <int M, int G, bool GUARD>
 __device__ __forceinline__ void flush_chunk(float4* __restrict__ dst, int strand0, int nstr, int N, int c,
-                                            const float4* __restrict__ stage, int lane) {
+                                            const float4* __restrict__ stage, int lane, int gsel = -1) {
   constexpr int C = Tile<M, G, GUARD>::C;
   // Recomputed here (opaque lane): keeping the load phase's addresses alive
   // until the stores would cost registers for the whole step.
@@ -130,7 +139,7 @@ __device__ __forceinline__ void flush_chunk(float4* __restrict__ dst, int strand
     const int j = q % C;
     const int s = strand0 + r / G;
     const int cp = (r % G) * M + c * C + j;
-    if (s < nstr && (!GUARD || cp < N)) dst[(size_t)s * N + cp] = stage[slot<C>(r, j)];
+    if (s < nstr && (!GUARD || cp < N) && (gsel < 0 || r % G == gsel)) dst[(size_t)s * N + cp] = stage[slot<C>(r, j)];
   }
   __syncthreads();
 }
@@ -229,9 +238,11 @@ __device__ __forceinline__ void collide_pred(const Spheres& S, float3& p, float3
 // iteration (the last G time steps); the earlier steps skip the velocity work.
 template <int M, int G, bool FINS>
 __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const Spheres& S, float3 (&p)[M], const LenRegs<M>& l,
-                                           float3 lead, int g, bool active, bool fin, bool valid,
-                                           const float4* __restrict__ pp, float4* __restrict__ vp, float3 u0,
-                                           float3& last_pre, float3& cfirst, float3& clast, unsigned cm) {
+                                           float3 lead, int g, bool active, bool fin, int gfin,
+                                           const float4* __restrict__ pp, float4* __restrict__ stage, int strand0,
+                                           int lane, float3 u0, float3& last_pre, float3& cfirst, float3& clast,
+                                           unsigned cm) {
+  constexpr int C = Tile<M, G, false>::C;
   float sg = A.sigma;
   asm volatile("" : "+v"(sg));
   const float dl = A.delta;
@@ -256,9 +267,20 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const Spheres& S, f
         if (j - 1 == 0) u = sel3(g == 0, u0, u);
         collide_pred(S, pm, u, active && !root, fin, cm);
         p[j - 1] = pm;
-        if (fin && valid) {
-          const float3 v = out_velocity(A, pp, j - 1, pm, u);
-          vp[j - 1] = make_float4(v.x, v.y, v.z, 0.0f);
+        // Final velocities go through the LDS stage; a completed chunk of the
+        // finishing lanes leaves coalesced.  The last chunk waits for the
+        // lane's last point (after the pipeline drains).
+        const int m = j - 1;
+        if constexpr (kStagedVel<M, G>) {
+          if (fin) {
+            const float3 v = out_velocity(A, pp, m, pm, u);
+            stage[slot<C>(lane, m % C)] = make_float4(v.x, v.y, v.z, 0.0f);
+          }
+          if (m % C == C - 1 && m / C < M / C - 1)
+            flush_chunk<M, G, false>(A.vel, strand0, A.nstrands, M * G, m / C, stage, lane, gfin);
+        } else if (fin && strand0 + lane / G < A.nstrands) {
+          const float3 v = out_velocity(A, pp, m, pm, u);
+          A.vel[(size_t)(strand0 + lane / G) * (M * G) + g * M + m] = make_float4(v.x, v.y, v.z, 0.0f);
         }
       }
       if (j == M - 1) clast = sel3(fin, c, clast);
@@ -423,21 +445,21 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
       }
     }
   } else {
-    float4* __restrict__ vp = A.vel + (size_t)(valid ? strand : 0) * N + (size_t)g * M;
     // No lane is in its last iteration before time step K - 1.
     for (int t = 0; t < K - 1; ++t) {
       const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
                            __shfl_up(last_pre.z, 1, G));
       const float3 lead = sel3(g == 0, p[0], li);
-      sweep_pipe<M, G, false>(A, S, p, l, lead, g, t >= g, false, valid, pp, vp, u0, last_pre, cfirst, clast, cm);
+      sweep_pipe<M, G, false>(A, S, p, l, lead, g, t >= g, false, -1, pp, stage, strand0, lane, u0, last_pre,
+                              cfirst, clast, cm);
     }
     for (int t = K - 1; t < K + G - 1; ++t) {
       const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
                            __shfl_up(last_pre.z, 1, G));
       const float3 lead = sel3(g == 0, p[0], li);
       const int it = t - g;
-      sweep_pipe<M, G, true>(A, S, p, l, lead, g, it >= 0 && it < K, it == K - 1, valid, pp, vp, u0, last_pre,
-                             cfirst, clast, cm);
+      sweep_pipe<M, G, true>(A, S, p, l, lead, g, it >= 0 && it < K, it == K - 1, t - (K - 1), pp, stage,
+                             strand0, lane, u0, last_pre, cfirst, clast, cm);
     }
     // The lane's last point: its velocity needs the next lane's correction.
     const float3 cn = f3(__shfl_down(cfirst.x, 1, G), __shfl_down(cfirst.y, 1, G),
@@ -447,7 +469,12 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     collide<true>(S, pm, u, cm);
     p[M - 1] = pm;
     const float3 v = out_velocity(A, pp, M - 1, pm, u);
-    if (valid) vp[M - 1] = make_float4(v.x, v.y, v.z, 0.0f);
+    if constexpr (kStagedVel<M, G>) {
+      stage[slot<C>(lane, (M - 1) % C)] = make_float4(v.x, v.y, v.z, 0.0f);
+      flush_chunk<M, G, GUARD>(A.vel, strand0, nstr, N, M / C - 1, stage, lane);
+    } else if (valid) {
+      A.vel[(size_t)strand * N + g * M + (M - 1)] = make_float4(v.x, v.y, v.z, 0.0f);
+    }
     bad = !(isfinite(pm.x) && isfinite(pm.y) && isfinite(pm.z) && isfinite(v.x) && isfinite(v.y) &&
             isfinite(v.z));
   }
