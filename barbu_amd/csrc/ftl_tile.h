@@ -65,6 +65,12 @@ struct Tile {
 template <int M, int G>
 constexpr bool kStagedVel = (M / Tile<M, G, false>::C) <= 2;
 
+// Pipelined forms with 16 points per lane update each point in place under
+// the lane mask (fewer selects and copies); the smaller forms keep selects,
+// which cost them fewer registers.
+template <int M>
+constexpr bool kInPlacePipe = (M == 16);
+
 // LDS slot of (owner lane r, point j of the chunk): XOR swizzle, conflict-free
 // for both the per-lane and the coalesced access order.
 template <int C>
@@ -250,13 +256,21 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const Spheres& S, f
 #pragma unroll
   for (int j = 0; j < M; ++j) {
     const float3 q = p[j];
-    const float3 d = sub3(q, lead);
-    const float s = (sg * l.get(j)) * rsqrt_c(dot3(d, d));
-    float3 pn = fma3(s, d, lead);
     const bool upd = active && (j != 0 || g != 0);  // lane 0's slot 0 is the root
-    pn = sel3(upd, pn, q);
-    p[j] = pn;
-    lead = pn;
+    float3 pn;
+    if constexpr (kInPlacePipe<M>) {
+      // The point is updated in its own registers under the lane mask; its
+      // leader is the previous point before that point's collision.
+      if (upd) ftl_point(p[j], j == 0 ? lead : p[j - 1], sg * l.get(j));
+      pn = p[j];
+    } else {
+      const float3 d = sub3(q, lead);
+      const float s = (sg * l.get(j)) * rsqrt_c(dot3(d, d));
+      pn = fma3(s, d, lead);
+      pn = sel3(upd, pn, q);
+      p[j] = pn;
+      lead = pn;
+    }
     if (FINS) {
       const float3 c = sub3(pn, q);
       if (j == 0) cfirst = sel3(fin, c, cfirst);
