@@ -42,7 +42,10 @@ def reference_params():
 def make_params(**kw):
     """Reference parameters with fields overridden by keyword."""
     p = reference_params()
+    names = {n for n, _ in Params._fields_}
     for k, v in kw.items():
+        if k not in names:
+            raise TypeError("unknown parameter %r" % k)
         if k in ("gravity", "wind_scroll"):
             for i in range(3):
                 getattr(p, k)[i] = float(v[i])
