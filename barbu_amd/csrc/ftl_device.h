@@ -26,7 +26,7 @@ struct FtlStep {
   const int4* bind;             // FTL_ROOT_SKINNED: {triangle, w1 bits, w2 bits, 0}
   const int4* tris;             // FTL_ROOT_SKINNED: {a, b, c, 0} per triangle
   const float4* skinned;        // FTL_ROOT_SKINNED: skinned vertices
-  const float4* wind;           // wind lattice, 2 float4 per node (layout at wind_at), or null
+  const float4* wind;           // wind lattice, 2 float4 per node (layout at wind_fetch), or null
   unsigned* status;             // bit 0: a strand tip became non-finite
   int nstrands, npoints, iterations, vmode;
   int rmode, ncol, model_ident, wind_grid;
@@ -132,38 +132,73 @@ __device__ __forceinline__ float lerp_c(float a, float b, float t) { return __bu
 // with d = v(x+1, y, z) - v(x, y, z) rounded once when the table is built, so
 // an x-edge is fma(s0, d, v): the same value as lerp(v, v(x+1), s0) computed
 // with the subtraction at run time.
-__device__ __forceinline__ float3 wind_edge(const float4* __restrict__ T, unsigned n, float s0) {
-  const float4 e0 = T[2u * n], e1 = T[2u * n + 1u];
+__device__ __forceinline__ float3 wind_edge(float4 e0, float2 e1, float s0) {
   return f3(__builtin_fmaf(s0, e0.w, e0.x), __builtin_fmaf(s0, e1.x, e0.y), __builtin_fmaf(s0, e1.y, e0.z));
 }
 
-__device__ __forceinline__ float3 wind_at(const FtlStep& A, float3 x) {
+// Wind parameters of one step, read once per tile.
+// A pointer re-formed from scalar registers is a generic pointer; holding it
+// as a global-address-space pointer keeps the gathers global loads (a flat
+// load also counts against the LDS counter, and its wait waits for both).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(1))) const float4 GlobalFloat4;
+#else
+typedef const float4 GlobalFloat4;
+#endif
+struct WindArgs {
+  GlobalFloat4* __restrict__ T;  // lattice (layout above), global memory
+  float freq;                    // lattice cells per unit length
+  float off[3];                  // lattice offset of this step
+};
+
+// The cell of x: smoothstep weights and the four x-edges, gathered together
+// (one round trip).  wind_fetch issues the gathers; wind_blend consumes them,
+// so a caller can place other memory operations in between.
+struct WindTaps {
+  float4 a[4];
+  float2 b[4];
+  float s0, s1, s2;
+};
+
+__device__ __forceinline__ WindTaps wind_fetch(const WindArgs& W, float3 x) {
   // The lattice side is the constant FTL_WIND_GRID (a power of two): corner
   // indices are masks and shifts, and every offset fits 32 bits, so the
   // gathers address the table from its scalar base.
   constexpr unsigned G = FTL_WIND_GRID, msk = FTL_WIND_GRID - 1;
-  const float u0 = __builtin_fmaf(x.x, A.wind_freq, A.wind_off[0]);
-  const float u1 = __builtin_fmaf(x.y, A.wind_freq, A.wind_off[1]);
-  const float u2 = __builtin_fmaf(x.z, A.wind_freq, A.wind_off[2]);
+  const float u0 = __builtin_fmaf(x.x, W.freq, W.off[0]);
+  const float u1 = __builtin_fmaf(x.y, W.freq, W.off[1]);
+  const float u2 = __builtin_fmaf(x.z, W.freq, W.off[2]);
   const float l0 = floorf(u0), l1 = floorf(u1), l2 = floorf(u2);
   const float f0 = u0 - l0, f1 = u1 - l1, f2 = u2 - l2;
   const unsigned i0 = (unsigned)(int)l0, i1 = (unsigned)(int)l1, i2 = (unsigned)(int)l2;
   const unsigned x0 = i0 & msk;
   const unsigned y0 = i1 & msk, y1 = (i1 + 1u) & msk;
   const unsigned z0 = i2 & msk, z1 = (i2 + 1u) & msk;
-  const float s0 = (f0 * f0) * __builtin_fmaf(-2.0f, f0, 3.0f);
-  const float s1 = (f1 * f1) * __builtin_fmaf(-2.0f, f1, 3.0f);
-  const float s2 = (f2 * f2) * __builtin_fmaf(-2.0f, f2, 3.0f);
-  const float4* __restrict__ T = A.wind;
-  // The four x-edges of the cell, gathered together (one round trip).
-  const float3 c00 = wind_edge(T, (z0 * G + y0) * G + x0, s0);
-  const float3 c10 = wind_edge(T, (z0 * G + y1) * G + x0, s0);
-  const float3 c01 = wind_edge(T, (z1 * G + y0) * G + x0, s0);
-  const float3 c11 = wind_edge(T, (z1 * G + y1) * G + x0, s0);
+  WindTaps t;
+  t.s0 = (f0 * f0) * __builtin_fmaf(-2.0f, f0, 3.0f);
+  t.s1 = (f1 * f1) * __builtin_fmaf(-2.0f, f1, 3.0f);
+  t.s2 = (f2 * f2) * __builtin_fmaf(-2.0f, f2, 3.0f);
+  const unsigned n[4] = {(z0 * G + y0) * G + x0, (z0 * G + y1) * G + x0, (z1 * G + y0) * G + x0,
+                         (z1 * G + y1) * G + x0};
+  const float4* __restrict__ T = (const float4*)W.T;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    t.a[e] = T[2u * n[e]];
+    const float4 h = T[2u * n[e] + 1u];
+    t.b[e] = make_float2(h.x, h.y);
+  }
+  return t;
+}
+
+__device__ __forceinline__ float3 wind_blend(const WindTaps& t) {
+  const float3 c00 = wind_edge(t.a[0], t.b[0], t.s0);
+  const float3 c10 = wind_edge(t.a[1], t.b[1], t.s0);
+  const float3 c01 = wind_edge(t.a[2], t.b[2], t.s0);
+  const float3 c11 = wind_edge(t.a[3], t.b[3], t.s0);
   float3 r;
-  r.x = lerp_c(lerp_c(c00.x, c10.x, s1), lerp_c(c01.x, c11.x, s1), s2);
-  r.y = lerp_c(lerp_c(c00.y, c10.y, s1), lerp_c(c01.y, c11.y, s1), s2);
-  r.z = lerp_c(lerp_c(c00.z, c10.z, s1), lerp_c(c01.z, c11.z, s1), s2);
+  r.x = lerp_c(lerp_c(c00.x, c10.x, t.s1), lerp_c(c01.x, c11.x, t.s1), t.s2);
+  r.y = lerp_c(lerp_c(c00.y, c10.y, t.s1), lerp_c(c01.y, c11.y, t.s1), t.s2);
+  r.z = lerp_c(lerp_c(c00.z, c10.z, t.s1), lerp_c(c01.z, c11.z, t.s1), t.s2);
   return r;
 }
 
@@ -179,6 +214,15 @@ __device__ __forceinline__ float pin_s(float x) {
   int i = __builtin_amdgcn_readfirstlane(__float_as_int(x));
   asm volatile("" : "+s"(i));
   return __int_as_float(i);
+}
+
+template <typename T>
+__device__ __forceinline__ T* pin_p(T* p) {
+  unsigned long long u = (unsigned long long)p;
+  int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)u);
+  int hi = __builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+  asm volatile("" : "+s"(lo), "+s"(hi));
+  return (T*)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
 __device__ __forceinline__ Spheres load_spheres(const FtlStep& A, unsigned cmask) {

@@ -29,6 +29,9 @@ constexpr int kWavesPerEu = FTL_WPE_OVERRIDE;
 #else
 constexpr int kWavesPerEu = CL::W;
 #endif
+// Load-phase prefetch depth: one row for the forms that target six or more
+// waves per SIMD (their register budget), FTL_LOAD_AHEAD for the others.
+constexpr int kLoadAhead = kWavesPerEu >= 6 ? 1 : FTL_LOAD_AHEAD;
 
 __global__ __launch_bounds__(FTL_WAVE) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
 void ftl_plain_kernel(FtlStep arg) {
@@ -42,7 +45,7 @@ void ftl_plain_kernel(FtlStep arg) {
   const FtlStep& A = arg;
 #endif
   __shared__ ftl_tile_impl::TileLds<CL::M, CL::G, CL::GU> sh;
-  ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU>(A, blockIdx.x * (FTL_WAVE / CL::G), sh);
+  ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead>(A, blockIdx.x * (FTL_WAVE / CL::G), sh);
 }
 
 }  // namespace
@@ -76,7 +79,7 @@ void ftl_tiles_kernel(const FtlStep* __restrict__ grooms, const FtlTile* __restr
     const FtlTile td = tiles[t];
     const int gi = __builtin_amdgcn_readfirstlane(td.groom);
     const int s0 = __builtin_amdgcn_readfirstlane(td.strand0);
-    ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU>(grooms[gi], s0, sh);
+    ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead>(grooms[gi], s0, sh);
   }
 }
 

@@ -44,6 +44,14 @@
 #ifndef FTL_WIND_ON
 #define FTL_WIND_ON 1
 #endif
+#ifndef FTL_PRIO_MODE
+#define FTL_PRIO_MODE 0
+#endif
+// Rows of the load phase whose loads are in flight ahead of the row being
+// integrated (default of the forms with at most four waves per SIMD).
+#ifndef FTL_LOAD_AHEAD
+#define FTL_LOAD_AHEAD 3
+#endif
 
 using namespace ftl_dev;
 
@@ -107,22 +115,21 @@ __device__ __forceinline__ float3 out_velocity(const FtlStep& A, const float4* _
   return v;
 }
 
-__device__ __forceinline__ float3 root_source(const FtlStep& A, int s, float4 x) {
+// Root source of strand s for the rest and skinned root modes (the model
+// mode's source is the root's own input position).
+__device__ __forceinline__ float3 root_source(const FtlStep& A, int s) {
   if (A.rmode == kRootRest) {
     const float4 r = A.roots[s];
     return f3(r.x, r.y, r.z);
   }
-  if (A.rmode == kRootSkinned) {
-    const int4 b = A.bind[s];
-    const int4 t = A.tris[b.x];
-    const float w1 = __int_as_float(b.y), w2 = __int_as_float(b.z);
-    const float w0 = (1.0f - w1) - w2;
-    const float4 a = A.skinned[t.x], bb = A.skinned[t.y], c = A.skinned[t.z];
-    return f3(__builtin_fmaf(w2, c.x, __builtin_fmaf(w1, bb.x, w0 * a.x)),
-              __builtin_fmaf(w2, c.y, __builtin_fmaf(w1, bb.y, w0 * a.y)),
-              __builtin_fmaf(w2, c.z, __builtin_fmaf(w1, bb.z, w0 * a.z)));
-  }
-  return f3(x.x, x.y, x.z);
+  const int4 b = A.bind[s];
+  const int4 t = A.tris[b.x];
+  const float w1 = __int_as_float(b.y), w2 = __int_as_float(b.z);
+  const float w0 = (1.0f - w1) - w2;
+  const float4 a = A.skinned[t.x], bb = A.skinned[t.y], c = A.skinned[t.z];
+  return f3(__builtin_fmaf(w2, c.x, __builtin_fmaf(w1, bb.x, w0 * a.x)),
+            __builtin_fmaf(w2, c.y, __builtin_fmaf(w1, bb.y, w0 * a.y)),
+            __builtin_fmaf(w2, c.z, __builtin_fmaf(w1, bb.z, w0 * a.z)));
 }
 
 // Writes chunk c of the tile (C points per lane, staged in LDS) to dst in
@@ -311,19 +318,22 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const Spheres& S, f
   }
 }
 
-// Shared memory of one tile: the load/store stage, and either the rest
-// lengths (one lane per strand) or the roots' velocities (pipelined forms).
+// Shared memory of one tile: the load/store stage, and the rest lengths of
+// the forms with one lane per strand.
 template <int M, int G, bool GUARD>
 struct TileLds {
   float4 stage[FTL_WAVE * Tile<M, G, GUARD>::C];
   float lens[G == 1 ? FTL_WAVE * M : 1];
-  float4 u0s[G == 1 ? 1 : FTL_WAVE];
+#ifdef FTL_LDS_PAD
+  float4 pad[FTL_LDS_PAD / 16];
+#endif
 };
 
-template <int M, int G, bool GUARD>
+// PD: rows of the load phase whose loads are in flight ahead of the row being
+// integrated (each costs 7 VGPRs; the forms with a high occupancy target use 1).
+template <int M, int G, bool GUARD, int PD_ = FTL_LOAD_AHEAD>
 __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<M, G, GUARD>& sh) {
   float4* __restrict__ stage = sh.stage;
-  float4* __restrict__ u0s = sh.u0s;
   constexpr int C = Tile<M, G, GUARD>::C;
   // Opaque: in the persistent kernels nothing lane-derived is hoisted out of
   // the tile loop (it would stay live, in registers, across every tile).
@@ -344,73 +354,94 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
   if constexpr (G == 1) l.base = sh.lens + lane;
 
   // ---- load + integrate, through LDS ------------------------------------
-  // Loads are clamped into the groom (out-of-range lanes read valid data that
-  // is never stored), so the loads of a group are issued back to back.
-  const float dt = A.dt, dt2 = A.dt2;
-  const float3 F0 = f3(A.force[0], A.force[1], A.force[2]);
+  // Row i of the tile is point i % C of chunk i / C of every owner lane, read
+  // in the coalesced order (64 consecutive float4 per instruction).  Loads run
+  // PD rows ahead of the row being integrated, and a row's wind gathers are
+  // issued before the loads of row i + PD: loads return in issue order, so
+  // waiting for the gathers never waits for HBM.  Loads are clamped into the
+  // groom (out-of-range lanes read valid data that is never stored).
+  const float dt = pin_s(A.dt), dt2 = pin_s(A.dt2);
+  const float3 F0 = f3(pin_s(A.force[0]), pin_s(A.force[1]), pin_s(A.force[2]));
+  const bool wind = FTL_WIND_ON && A.wind_grid;
+  WindArgs W;
+  W.T = (GlobalFloat4*)pin_p(A.wind);
+  W.freq = pin_s(A.wind_freq);
+  W.off[0] = pin_s(A.wind_off[0]);
+  W.off[1] = pin_s(A.wind_off[1]);
+  W.off[2] = pin_s(A.wind_off[2]);
+  // The root source of the lane's strand, fetched first so that its latency
+  // overlaps the row loads.  The root's row slot carries its input position
+  // through the transpose; the root is formed from both after it.
+  const bool owns_root = (G == 1 || g == 0);
+  float3 rsrc = f3(0.f, 0.f, 0.f);
+  if (owns_root && A.rmode != kRootModel) rsrc = root_source(A, min(strand, slast));
+  __builtin_amdgcn_sched_barrier(0);
+
+  constexpr int PD = PD_ < M ? PD_ : M;
+  float4 xr[PD + 1], vr[PD + 1];
+  auto row_index = [&](int i) -> size_t {
+    const int q = (i % C) * FTL_WAVE + lane;
+    const int r = q / C;
+    const int s = min(strand0 + r / G, slast);
+    int cp = (r % G) * M + (i / C) * C + (q % C);
+    if (GUARD) cp = min(cp, N - 1);
+    return (size_t)s * N + cp;
+  };
+  auto load_row = [&](int i) {
+    const size_t idx = row_index(i);
+    xr[i % (PD + 1)] = A.pos[idx];
+    vr[i % (PD + 1)] = A.vel[idx];
+  };
 #pragma unroll
-  for (int c = 0; c < M / C; ++c) {
-    // Groups of H points: 2H loads in flight, then the points are integrated
-    // one after another (the scheduling barrier keeps the register peak of the
-    // wind gathers to one point).
-    constexpr int H = C < 2 ? C : 2;
-#pragma unroll
-    for (int k0 = 0; k0 < C; k0 += H) {
-      float4 xs[H], vs[H];
-#pragma unroll
-      for (int h = 0; h < H; ++h) {
-        const int q = (k0 + h) * FTL_WAVE + lane;
-        const int r = q / C;  // owner lane
-        const int s = min(strand0 + r / G, slast);
-        int cp = (r % G) * M + c * C + (q % C);
-        if (GUARD) cp = min(cp, N - 1);
-        const size_t idx = (size_t)s * N + cp;
-        xs[h] = A.pos[idx];
-        vs[h] = A.vel[idx];
-      }
-#pragma unroll
-      for (int h = 0; h < H; ++h) {
-        const int q = (k0 + h) * FTL_WAVE + lane;
-        const int r = q / C;
-        const int j = q % C;
-        const int cp = (r % G) * M + c * C + j;
-        const float4 x = xs[h], v = vs[h];
-        float3 F = F0;
-        if (FTL_WIND_ON && A.wind_grid) {
-          const float3 w = wind_at(A, f3(x.x, x.y, x.z));
-          F = f3(F0.x + w.x, F0.y + w.y, F0.z + w.z);
-        }
-        float3 pr = f3(__builtin_fmaf(dt2, F.x, __builtin_fmaf(dt, v.x, x.x)),
-                       __builtin_fmaf(dt2, F.y, __builtin_fmaf(dt, v.y, x.y)),
-                       __builtin_fmaf(dt2, F.z, __builtin_fmaf(dt, v.z, x.z)));
-        if (cp == 0) {
-          const float3 src = root_source(A, min(strand0 + r / G, slast), x);
-          pr = A.model_ident ? src : transform(A, src);
-          if (G > 1) {
-            const float3 u = sub3(pr, f3(x.x, x.y, x.z));
-            u0s[r] = make_float4(u.x, u.y, u.z, 0.0f);
-          }
-        }
-        stage[slot<C>(r, j)] = make_float4(pr.x, pr.y, pr.z, x.w);
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < C; ++j) {
-      const float4 t = stage[slot<C>(lane, j)];
-      p[c * C + j] = f3(t.x, t.y, t.z);
-      l.set(c * C + j, t.w);
-    }
-    __syncthreads();
-  }
+  for (int i = 0; i < PD; ++i) load_row(i);
   float3 u0 = f3(0.f, 0.f, 0.f);
-  if (G > 1 && g == 0) {
-    const float4 t = u0s[lane];
-    u0 = f3(t.x, t.y, t.z);
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int c = i / C;
+    const int q = (i % C) * FTL_WAVE + lane;
+    const int r = q / C;
+    const int j = q % C;
+    const int cp = (r % G) * M + c * C + j;
+    const float4 x = xr[i % (PD + 1)], v = vr[i % (PD + 1)];
+    WindTaps wt;
+    if (wind) wt = wind_fetch(W, f3(x.x, x.y, x.z));
+    __builtin_amdgcn_sched_barrier(0);
+    if (i + PD < M) load_row(i + PD);
+    __builtin_amdgcn_sched_barrier(0);
+    float3 F = F0;
+    if (wind) {
+      const float3 w = wind_blend(wt);
+      F = f3(F0.x + w.x, F0.y + w.y, F0.z + w.z);
+    }
+    float3 pr = f3(__builtin_fmaf(dt2, F.x, __builtin_fmaf(dt, v.x, x.x)),
+                   __builtin_fmaf(dt2, F.y, __builtin_fmaf(dt, v.y, x.y)),
+                   __builtin_fmaf(dt2, F.z, __builtin_fmaf(dt, v.z, x.z)));
+    if (cp == 0) pr = f3(x.x, x.y, x.z);
+    stage[slot<C>(r, j)] = make_float4(pr.x, pr.y, pr.z, x.w);
+    __builtin_amdgcn_sched_barrier(0);
+    if (i % C == C - 1) {
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        const float4 t = stage[slot<C>(lane, k)];
+        p[c * C + k] = f3(t.x, t.y, t.z);
+        l.set(c * C + k, t.w);
+      }
+      __syncthreads();
+      if (c == 0 && owns_root) {
+        // root: p0 = M * source (:190-193); the pipelined forms keep u0 = p0 - x0
+        const float3 x0 = p[0];
+        const float3 src = A.rmode == kRootModel ? x0 : rsrc;
+        p[0] = A.model_ident ? src : transform(A, src);
+        if (G > 1) u0 = sub3(p[0], x0);
+      }
+    }
   }
 
   // ---- colliders the wave can reach this step ----------------------------
+#if FTL_PRIO_MODE == 1
+  __builtin_amdgcn_s_setprio(0);
+#endif
   unsigned cm;
   {
     float L = 0.0f;
@@ -493,6 +524,9 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
             isfinite(v.z));
   }
   // positions, after every velocity (DFTL reads the step's input positions)
+#if FTL_PRIO_MODE == 1
+  __builtin_amdgcn_s_setprio(2);
+#endif
 #pragma unroll
   for (int c = 0; c < M / C; ++c) {
     if (GUARD && c * C >= nl) continue;
