@@ -44,9 +44,6 @@
 #ifndef FTL_WIND_ON
 #define FTL_WIND_ON 1
 #endif
-#ifndef FTL_PRIO_MODE
-#define FTL_PRIO_MODE 0
-#endif
 // Rows of the load phase whose loads are in flight ahead of the row being
 // integrated (default of the forms with at most four waves per SIMD).
 #ifndef FTL_LOAD_AHEAD
@@ -319,14 +316,18 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const Spheres& S, f
 }
 
 // Shared memory of one tile: the load/store stage, and the rest lengths of
-// the forms with one lane per strand.
+// the forms with one lane per strand (an empty base for the others, so a
+// pipelined tile takes exactly its stage).
+template <int M, bool ON>
+struct TileLens {
+  float lens[FTL_WAVE * M];
+};
+template <int M>
+struct TileLens<M, false> {};
+
 template <int M, int G, bool GUARD>
-struct TileLds {
+struct TileLds : TileLens<M, G == 1> {
   float4 stage[FTL_WAVE * Tile<M, G, GUARD>::C];
-  float lens[G == 1 ? FTL_WAVE * M : 1];
-#ifdef FTL_LDS_PAD
-  float4 pad[FTL_LDS_PAD / 16];
-#endif
 };
 
 // PD: rows of the load phase whose loads are in flight ahead of the row being
@@ -439,9 +440,6 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
   }
 
   // ---- colliders the wave can reach this step ----------------------------
-#if FTL_PRIO_MODE == 1
-  __builtin_amdgcn_s_setprio(0);
-#endif
   unsigned cm;
   {
     float L = 0.0f;
@@ -524,9 +522,6 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
             isfinite(v.z));
   }
   // positions, after every velocity (DFTL reads the step's input positions)
-#if FTL_PRIO_MODE == 1
-  __builtin_amdgcn_s_setprio(2);
-#endif
 #pragma unroll
   for (int c = 0; c < M / C; ++c) {
     if (GUARD && c * C >= nl) continue;
