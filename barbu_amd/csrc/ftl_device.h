@@ -216,6 +216,12 @@ __device__ __forceinline__ float pin_s(float x) {
   return __int_as_float(i);
 }
 
+__device__ __forceinline__ int pin_i(int x) {
+  int i = __builtin_amdgcn_readfirstlane(x);
+  asm volatile("" : "+s"(i));
+  return i;
+}
+
 template <typename T>
 __device__ __forceinline__ T* pin_p(T* p) {
   unsigned long long u = (unsigned long long)p;

@@ -98,17 +98,35 @@ struct LenLds {
   __device__ __forceinline__ void set(int j, float x) { base[j * FTL_WAVE] = x; }
 };
 
+// Parameters of the final iteration's velocities, read once per tile into
+// scalar registers (reloading them point by point costs a scalar-memory wait
+// each).
+struct VelArgs {
+  int dftl;  // DFTL Eq. 9 instead of the shipped correction
+  float inv_dt, maxv, maxv2, delta;
+};
+
+__device__ __forceinline__ VelArgs vel_args(const FtlStep& A) {
+  VelArgs V;
+  V.dftl = pin_i(!FTL_NO_DFTL && A.vmode == kVelDftl);
+  V.inv_dt = pin_s(A.inv_dt);
+  V.maxv = pin_s(A.maxv);
+  V.maxv2 = pin_s(A.maxv2);
+  V.delta = pin_s(A.delta);
+  return V;
+}
+
 // Stored velocity of a final point: the damped, reflected correction u as
 // shipped, or DFTL Eq. 9 with VelocityControl (:79-84, :201-204).  In DFTL
 // mode x is re-read from memory, where the point's position is still the
 // step's input (positions are written after all velocities).
-__device__ __forceinline__ float3 out_velocity(const FtlStep& A, const float4* __restrict__ pp, int j, float3 p,
+__device__ __forceinline__ float3 out_velocity(const VelArgs& V, const float4* __restrict__ pp, int j, float3 p,
                                                float3 u) {
-  if (FTL_NO_DFTL || A.vmode != kVelDftl) return u;
+  if (FTL_NO_DFTL || !V.dftl) return u;
   const float4 x = pp[j];
-  float3 v = f3(((p.x - x.x) - u.x) * A.inv_dt, ((p.y - x.y) - u.y) * A.inv_dt, ((p.z - x.z) - u.z) * A.inv_dt);
+  float3 v = f3(((p.x - x.x) - u.x) * V.inv_dt, ((p.y - x.y) - u.y) * V.inv_dt, ((p.z - x.z) - u.z) * V.inv_dt);
   const float dp = dot3(v, v);
-  if (dp > A.maxv2) v = mul3(v, A.maxv * rsqrt_c(dp));
+  if (dp > V.maxv2) v = mul3(v, V.maxv * rsqrt_c(dp));
   return v;
 }
 
@@ -160,7 +178,8 @@ __device__ __forceinline__ void flush_chunk(float4* __restrict__ dst, int strand
 // the lane's last point is left to the caller (its velocity is the raw
 // correction and its collision comes after the sweep).
 template <int M, bool GUARD, bool FINAL>
-__device__ __forceinline__ void sweep(const FtlStep& A, const Spheres& S, float3 (&p)[M], const LenLds& l, int nl,
+__device__ __forceinline__ void sweep(const FtlStep& A, const VelArgs& V, const Spheres& S, float3 (&p)[M],
+                                      const LenLds& l, int nl,
                                       const float4* __restrict__ pp, float3& last_pre, float3& clast,
                                       float4* __restrict__ stage, int strand0, int N, int lane, unsigned cm) {
   constexpr int C = Tile<M, 1, GUARD>::C;
@@ -168,7 +187,7 @@ __device__ __forceinline__ void sweep(const FtlStep& A, const Spheres& S, float3
   // iteration loop into M extra registers (one multiply per point instead).
   float sg = A.sigma;
   asm volatile("" : "+v"(sg));
-  const float dl = A.delta;
+  const float dl = V.delta;
 #pragma unroll
   for (int j = 1; j < M; ++j) {
     // GUARD: slots j >= nl hold copies of the last point; they are swept like
@@ -192,7 +211,7 @@ __device__ __forceinline__ void sweep(const FtlStep& A, const Spheres& S, float3
           collide<true>(S, pm, u, cm);
         }
         p[m] = pm;
-        const float3 v = out_velocity(A, pp, m, pm, u);
+        const float3 v = out_velocity(V, pp, m, pm, u);
         stage[slot<C>(lane, m % C)] = make_float4(v.x, v.y, v.z, 0.0f);
         if (m % C == C - 1) flush_chunk<M, 1, GUARD>(A.vel, strand0, A.nstrands, N, m / C, stage, lane);
       }
@@ -247,7 +266,8 @@ __device__ __forceinline__ void collide_pred(const Spheres& S, float3& p, float3
 // the two sides of a divergent branch.  FINS: some lane may be in its last
 // iteration (the last G time steps); the earlier steps skip the velocity work.
 template <int M, int G, bool FINS>
-__device__ __forceinline__ void sweep_pipe(const FtlStep& A, const Spheres& S, float3 (&p)[M], const LenRegs<M>& l,
+__device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, const Spheres& S, float3 (&p)[M],
+                                           const LenRegs<M>& l,
                                            float3 lead, int g, bool active, bool fin, int gfin,
                                            const float4* __restrict__ pp, float4* __restrict__ stage, int strand0,
                                            int lane, float3 u0, float3& last_pre, float3& cfirst, float3& clast,
@@ -255,14 +275,14 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const Spheres& S, f
   constexpr int C = Tile<M, G, false>::C;
   float sg = A.sigma;
   asm volatile("" : "+v"(sg));
-  const float dl = A.delta;
+  const float dl = V.delta;
   if (!FINS) fin = false;
 #pragma unroll
   for (int j = 0; j < M; ++j) {
     const float3 q = p[j];
     const bool upd = active && (j != 0 || g != 0);  // lane 0's slot 0 is the root
     float3 pn;
-    if constexpr (kInPlacePipe<M>) {
+    if constexpr (kInPlacePipe<M> && !FINS) {
       // The point is updated in its own registers under the lane mask; its
       // leader is the previous point before that point's collision.
       if (upd) ftl_point(p[j], j == 0 ? lead : p[j - 1], sg * l.get(j));
@@ -291,13 +311,13 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const Spheres& S, f
         const int m = j - 1;
         if constexpr (kStagedVel<M, G>) {
           if (fin) {
-            const float3 v = out_velocity(A, pp, m, pm, u);
+            const float3 v = out_velocity(V, pp, m, pm, u);
             stage[slot<C>(lane, m % C)] = make_float4(v.x, v.y, v.z, 0.0f);
           }
           if (m % C == C - 1 && m / C < M / C - 1)
             flush_chunk<M, G, false>(A.vel, strand0, A.nstrands, M * G, m / C, stage, lane, gfin);
         } else if (fin && strand0 + lane / G < A.nstrands) {
-          const float3 v = out_velocity(A, pp, m, pm, u);
+          const float3 v = out_velocity(V, pp, m, pm, u);
           A.vel[(size_t)(strand0 + lane / G) * (M * G) + g * M + m] = make_float4(v.x, v.y, v.z, 0.0f);
         }
       }
@@ -459,6 +479,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     cm = __builtin_amdgcn_readfirstlane(any);
   }
   const Spheres S = load_spheres(A, cm);
+  const VelArgs V = vel_args(A);
 
   // ---- constraints ------------------------------------------------------
   // Lane's own points, for the DFTL re-read of x (never stored through).
@@ -470,8 +491,8 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
   bool bad = false;
   if constexpr (G == 1) {
     for (int it = 0; it < K - 1; ++it)
-      sweep<M, GUARD, false>(A, S, p, l, nl, pp, last_pre, clast, stage, strand0, N, lane, cm);
-    sweep<M, GUARD, true>(A, S, p, l, nl, pp, last_pre, clast, stage, strand0, N, lane, cm);
+      sweep<M, GUARD, false>(A, V, S, p, l, nl, pp, last_pre, clast, stage, strand0, N, lane, cm);
+    sweep<M, GUARD, true>(A, V, S, p, l, nl, pp, last_pre, clast, stage, strand0, N, lane, cm);
     // The last point: velocity = raw correction, collisions deferred to here.
     u = clast;
 #pragma unroll
@@ -480,7 +501,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
         float3 pm = p[j];
         collide<true>(S, pm, u, cm);
         p[j] = pm;
-        const float3 v = out_velocity(A, pp, j, pm, u);
+        const float3 v = out_velocity(V, pp, j, pm, u);
         bad = !(isfinite(pm.x) && isfinite(pm.y) && isfinite(pm.z) && isfinite(v.x) && isfinite(v.y) &&
                 isfinite(v.z));
         stage[slot<C>(lane, j % C)] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -493,7 +514,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
       const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
                            __shfl_up(last_pre.z, 1, G));
       const float3 lead = sel3(g == 0, p[0], li);
-      sweep_pipe<M, G, false>(A, S, p, l, lead, g, t >= g, false, -1, pp, stage, strand0, lane, u0, last_pre,
+      sweep_pipe<M, G, false>(A, V, S, p, l, lead, g, t >= g, false, -1, pp, stage, strand0, lane, u0, last_pre,
                               cfirst, clast, cm);
     }
     for (int t = K - 1; t < K + G - 1; ++t) {
@@ -501,17 +522,17 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
                            __shfl_up(last_pre.z, 1, G));
       const float3 lead = sel3(g == 0, p[0], li);
       const int it = t - g;
-      sweep_pipe<M, G, true>(A, S, p, l, lead, g, it >= 0 && it < K, it == K - 1, t - (K - 1), pp, stage,
+      sweep_pipe<M, G, true>(A, V, S, p, l, lead, g, it >= 0 && it < K, it == K - 1, t - (K - 1), pp, stage,
                              strand0, lane, u0, last_pre, cfirst, clast, cm);
     }
     // The lane's last point: its velocity needs the next lane's correction.
     const float3 cn = f3(__shfl_down(cfirst.x, 1, G), __shfl_down(cfirst.y, 1, G),
                          __shfl_down(cfirst.z, 1, G));
-    u = sel3(g == G - 1, clast, mul3(cn, A.delta));
+    u = sel3(g == G - 1, clast, mul3(cn, V.delta));
     float3 pm = p[M - 1];
     collide<true>(S, pm, u, cm);
     p[M - 1] = pm;
-    const float3 v = out_velocity(A, pp, M - 1, pm, u);
+    const float3 v = out_velocity(V, pp, M - 1, pm, u);
     if constexpr (kStagedVel<M, G>) {
       stage[slot<C>(lane, (M - 1) % C)] = make_float4(v.x, v.y, v.z, 0.0f);
       flush_chunk<M, G, GUARD>(A.vel, strand0, nstr, N, M / C - 1, stage, lane);
