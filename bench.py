@@ -57,12 +57,14 @@ def context(ba):
     return ctx
 
 
-def bench_single(ba, hipm, steps, warmup, S=262144, N=32):
+def bench_single(ba, hipm, steps, warmup, S=262144, N=32, lanes=None):
     pos, vel, roots = ba.sphere_groom(S, N, radius=1.0, length=0.2, jitter=0.05, seed=2024)
     ctx = context(ba)
     g = ba.Groom(ctx, pos, vel, rest_roots=roots)
     g.set_params(c2_params(ba))
     g.set_root_mode(ba.FTL_ROOT_REST)
+    if lanes:
+        g.set_lanes(lanes)
     stream = hipm.Stream()
     e0, e1 = hipm.Event(), hipm.Event()
 
@@ -87,7 +89,7 @@ def bench_single(ba, hipm, steps, warmup, S=262144, N=32):
     gpu_ms = e0.elapsed_ms(e1)
     p, v = g.download()
     assert np.isfinite(p).all() and np.isfinite(v).all(), "non-finite state after the benchmark"
-    lanes = ctx.lanes_for(N)
+    lanes = lanes or ctx.lanes_for(N)
     g.close()
     copy_gbs = copy_reference(ba, hipm, ctx, S * N * 32)
     ctx.close()
@@ -470,6 +472,8 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=4.0)
     ap.add_argument("--skinned-steps", type=int, default=120)
     ap.add_argument("--batched-frames", type=int, default=60)
+    ap.add_argument("--lanes", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="lanes per strand of the single-groom workload (0: the tuned table)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (strands and grooms shard with no "
@@ -482,7 +486,7 @@ def main():
     if args.workload == "batched":
         args.steps = max(args.steps, 1)
     log("single groom: 262144 x 32, %d warmup + %d timed steps" % (args.warmup, args.steps))
-    r = bench_single(ba, hipm, args.steps, args.warmup)
+    r = bench_single(ba, hipm, args.steps, args.warmup, lanes=args.lanes or None)
     cps = r["S"] * r["N"]
     ms_per_step = 1000.0 * r["wall_s"] / args.steps
     value = cps * args.steps / r["wall_s"]
