@@ -17,6 +17,7 @@ from ._ffi import (FTL_ROOT_MODEL, FTL_ROOT_REST, FTL_ROOT_SKINNED, FTL_VELOCITY
 
 __all__ = ["Context", "Groom", "Skin", "Batch", "Server", "make_requests", "set_request", "Params", "reference_params", "FtlError",
            "sphere_groom", "reference_groom", "skinned_scene", "bones", "wind_table",
+           "Expand", "expand_params", "strand_tangents", "random_table",
            "FTL_ROOT_MODEL", "FTL_ROOT_REST", "FTL_ROOT_SKINNED", "FTL_VELOCITY_SHIPPED",
            "FTL_VELOCITY_DFTL", "Hair"]
 
@@ -396,7 +397,122 @@ class Server:
             pass
 
 
+class DeviceBuffer:
+    """Device memory owned by the context (ftl_diag_alloc), for outputs such as
+    the expansion's vertices."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx, self.nbytes = ctx, int(nbytes)
+        self.ptr = C.c_void_p()
+        check(lib().ftl_diag_alloc(ctx.handle, self.nbytes, C.byref(self.ptr)), "ftl_diag_alloc")
+
+    def download(self, dtype=np.float32, shape=None):
+        from . import _hip
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        _hip.copy_to_host(out, self.ptr)
+        return out if shape is None else out.reshape(shape)
+
+    def close(self):
+        if self.ptr:
+            lib().ftl_diag_free(self.ctx.handle, self.ptr, self.nbytes)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def expand_params(ninstances=None, nlines=None, nsubsegments=None, tangent_scale=None):
+    """Tessellation settings of the stream pass; unset fields keep the reference defaults."""
+    p = _ffi.ExpandParams()
+    lib().ftl_expand_params_reference(C.byref(p))
+    if ninstances is not None:
+        p.ninstances = int(ninstances)
+    if nlines is not None:
+        p.nlines = int(nlines)
+    if nsubsegments is not None:
+        p.nsubsegments = int(nsubsegments)
+    if tangent_scale is not None:
+        p.tangent_scale = float(tangent_scale)
+    return p
+
+
+class Expand:
+    """Render-side strand expansion (ftl_expand_*, include/ftl_render.h): the
+    tessellation stream pass of the reference renderer on the device."""
+
+    def __init__(self, ctx, nstrands, npoints, tris, tangents, random):
+        self.ctx = ctx
+        self.nstrands, self.npoints = int(nstrands), int(npoints)
+        self.tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        tangents = _f32(tangents).reshape(self.nstrands * self.npoints, 4)
+        self.random = _f32(random).reshape(-1, 4)
+        self._h = C.c_void_p()
+        check(lib().ftl_expand_create(ctx.handle, self.nstrands, self.npoints, _ptr(self.tris), self.tris.shape[0],
+                                      _ptr(tangents), _ptr(self.random), self.random.shape[0], C.byref(self._h)),
+              "ftl_expand_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_tangents(self, tangents):
+        tangents = _f32(tangents).reshape(self.nstrands * self.npoints, 4)
+        check(lib().ftl_expand_set_tangents(self._h, _ptr(tangents)), "ftl_expand_set_tangents")
+
+    def vertex_count(self, params=None):
+        p = params or expand_params()
+        return int(lib().ftl_expand_vertex_count(self._h, C.byref(p)))
+
+    def run(self, positions, out, capacity, params=None, stream=None):
+        """positions, out: device pointers; capacity in vertices."""
+        p = params or expand_params()
+        check(lib().ftl_expand_run(self._h, C.byref(p), positions, out, int(capacity), stream), "ftl_expand_run")
+
+    def run_groom(self, groom, params=None, stream=None):
+        """Expands a groom's current positions; returns float32 [vertices, 4]."""
+        from . import _hip
+        n = self.vertex_count(params)
+        buf = DeviceBuffer(self.ctx, max(n, 1) * 16)
+        try:
+            pos, _ = groom.device_buffers()
+            self.run(pos, buf.ptr, n, params, stream)
+            _hip.device_synchronize()
+            return buf.download(np.float32, (n, 4))
+        finally:
+            buf.close()
+
+    def close(self):
+        if self._h:
+            lib().ftl_expand_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # --- scenario generators (CPU) ----------------------------------------------
+
+def strand_tangents(pos):
+    """Central-difference tangents of strands [S, N, 4] (ftl_scn_strand_tangents)."""
+    pos = _f32(pos)
+    S, N = pos.shape[0], pos.shape[1]
+    out = np.empty_like(pos)
+    check(lib().ftl_scn_strand_tangents(_ptr(pos), S, N, _ptr(out)), "ftl_scn_strand_tangents")
+    return out
+
+
+def random_table(seed=0, n=4096):
+    """The renderer's random table, seeded (ftl_scn_random_table): float32 [n, 4]."""
+    out = np.empty((n, 4), np.float32)
+    check(lib().ftl_scn_random_table(int(seed), int(n), _ptr(out)), "ftl_scn_random_table")
+    return out
+
 
 def sphere_groom(nstrands, npoints, center=(0, 0, 0), radius=1.0, length=0.2, jitter=0.05, seed=0,
                  coherent=True):

@@ -1,4 +1,4 @@
-"""ctypes declarations of libftl.so (include/ftl.h, include/ftl_scenario.h)."""
+"""ctypes declarations of libftl.so (include/ftl.h, ftl_scenario.h, ftl_server.h, ftl_render.h)."""
 import ctypes as C
 import os
 
@@ -38,6 +38,15 @@ class Params(C.Structure):
         ("wind_frequency", C.c_float),
         ("wind_scroll", C.c_float * 3),
         ("wind_seed", C.c_uint64),
+    ]
+
+
+class ExpandParams(C.Structure):
+    _fields_ = [
+        ("ninstances", C.c_int),
+        ("nlines", C.c_int),
+        ("nsubsegments", C.c_int),
+        ("tangent_scale", C.c_float),
     ]
 
 
@@ -192,6 +201,14 @@ SIGNATURES = {
     "ftl_server_status": (st, [vp, C.c_int, ip]),
     "ftl_server_get_stats": (st, [vp, C.POINTER(ServerStats)]),
     "ftl_server_reset_stats": (st, [vp]),
+    "ftl_scn_strand_tangents": (st, [vp, C.c_int, C.c_int, vp]),
+    "ftl_scn_random_table": (st, [C.c_uint64, C.c_int, vp]),
+    "ftl_expand_params_reference": (None, [C.POINTER(ExpandParams)]),
+    "ftl_expand_create": (st, [vp, C.c_int, C.c_int, vp, C.c_int, vp, vp, C.c_int, pp]),
+    "ftl_expand_destroy": (None, [vp]),
+    "ftl_expand_set_tangents": (st, [vp, vp]),
+    "ftl_expand_vertex_count": (C.c_int64, [vp, C.POINTER(ExpandParams)]),
+    "ftl_expand_run": (st, [vp, C.POINTER(ExpandParams), vp, vp, C.c_int64, vp]),
 }
 
 _lib = None

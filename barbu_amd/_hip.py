@@ -16,6 +16,7 @@ def hip():
         h.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
         h.hipStreamSynchronize.argtypes = [C.c_void_p]
         h.hipStreamDestroy.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _hip = h
     return _hip
 
@@ -40,6 +41,11 @@ class Event:
         ms = C.c_float(0)
         _ok(hip().hipEventElapsedTime(C.byref(ms), self.h, end.h), "hipEventElapsedTime")
         return ms.value
+
+
+def copy_to_host(dst_array, src_device_ptr):
+    """Blocking device-to-host copy into a contiguous numpy array."""
+    _ok(hip().hipMemcpy(C.c_void_p(dst_array.ctypes.data), src_device_ptr, dst_array.nbytes, 2), "hipMemcpy")
 
 
 def device_synchronize():

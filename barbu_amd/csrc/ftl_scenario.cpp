@@ -300,3 +300,30 @@ extern "C" ftl_status ftl_scn_bones(const ftl_scn_skinned* d, double t, float* d
   }
   return FTL_OK;
 }
+
+extern "C" ftl_status ftl_scn_strand_tangents(const float* pos, int nstrands, int npoints, float* tangents) {
+  if (!pos || !tangents || nstrands <= 0 || npoints < 2)
+    return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "bad strand tangent arguments");
+  for (int s = 0; s < nstrands; ++s) {
+    const float* p = pos + (size_t)4 * npoints * s;
+    float* t = tangents + (size_t)4 * npoints * s;
+    for (int i = 0; i < npoints; ++i) {
+      const int a = i > 0 ? i - 1 : 0, b = i < npoints - 1 ? i + 1 : npoints - 1;
+      const float k = (b - a == 2) ? 0.5f : 1.0f;
+      for (int c = 0; c < 3; ++c) t[4 * i + c] = (p[4 * b + c] - p[4 * a + c]) * k;
+      t[4 * i + 3] = 0.0f;
+    }
+  }
+  return FTL_OK;
+}
+
+extern "C" ftl_status ftl_scn_random_table(uint64_t seed, int n, float* out) {
+  if (!out || n <= 0) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "bad random table arguments");
+  Rng rng(seed);
+  for (int i = 0; i < n; ++i) {
+    for (int c = 0; c < 3; ++c) out[4 * (size_t)i + c] = (float)(rng.next() >> 40) * 0x1p-24f;  // [0, 1)
+    out[4 * (size_t)i + 3] = 0.0f;
+  }
+  return FTL_OK;
+}
+

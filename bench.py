@@ -448,6 +448,50 @@ def bench_skinned(ba, hipm, steps, warmup, S=4_194_304, N=32):
     return dict(S=S, N=N, nverts=nverts, wall_s=wall, gpu_ms=gpu_ms, lanes=lanes, nonfinite=bool(nonfinite))
 
 
+def bench_expand(ba, hipm, reps, warmup, ntris=100_000, npoints=4, seed=5):
+    """Render-side strand expansion (the stream pass of the reference renderer):
+    one guide strand per vertex of a 100k-triangle scalp (the lat/long sphere
+    mesh of the skinned scene), initialised as Hair::init_simulation does with
+    the reference's 4 control points, central-difference tangents, a seeded
+    random table and the reference tessellation settings (3 instances, 2
+    lines, 16 subsegments)."""
+    sc = ba.skinned_scene(1, npoints, ntris=ntris, nbones=4, seed=seed)
+    roots = sc["rest"][:, :3].astype(np.float64)
+    normals = roots / np.linalg.norm(roots, axis=1, keepdims=True)
+    pos, vel = ba.reference_groom(roots, normals, npoints=npoints, maxlength=0.2, seed=seed)
+    tan = ba.strand_tangents(pos)
+    rnd = ba.random_table(seed, 4096)
+    ctx = context(ba)
+    g = ba.Groom(ctx, pos, vel)
+    e = ba.Expand(ctx, pos.shape[0], npoints, sc["tris"], tan, rnd)
+    p = ba.expand_params()
+    n = e.vertex_count(p)
+    buf = ba.DeviceBuffer(ctx, n * 16)
+    ptr, _ = g.device_buffers()
+    stream = hipm.Stream()
+    e0, e1 = hipm.Event(), hipm.Event()
+    for _ in range(warmup):
+        e.run(ptr, buf.ptr, n, p, stream.h)
+    hipm.device_synchronize()
+    e0.record(stream.h)
+    for _ in range(reps):
+        e.run(ptr, buf.ptr, n, p, stream.h)
+    e1.record(stream.h)
+    hipm.device_synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    head = buf.download(np.float32)[:8 * 4]
+    finite = bool(np.isfinite(head).all())
+    ntri = sc["tris"].shape[0]
+    written = n * 16
+    read = pos.shape[0] * npoints * 32 + ntri * 16 + rnd.nbytes
+    buf.close()
+    e.close()
+    g.close()
+    ctx.close()
+    return dict(ntris=ntri, strands=pos.shape[0], npoints=npoints, vertices=n, ms=ms, written=written, read=read,
+                finite=finite)
+
+
 def load_traffic(workload):
     """HBM bytes per launch of the workload's kernel, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE runs summarised in profiles/traffic.json (None if
@@ -468,7 +512,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--workload", default="all", choices=["single", "batched", "served", "skinned", "all"])
+    ap.add_argument("--workload", default="all",
+                    choices=["single", "batched", "served", "skinned", "expand", "all"])
     ap.add_argument("--served-seconds", type=float, default=4.0)
     ap.add_argument("--skinned-steps", type=int, default=120)
     ap.add_argument("--batched-frames", type=int, default=60)
@@ -560,6 +605,18 @@ def main():
             "gpu_ms_per_frame": k["gpu_ms"] / args.skinned_steps,
             "achieved_gbs": kgb, "frac_hbm_peak": kgb / HBM_PEAK_GBS, "lanes_per_strand": k["lanes"],
             "nonfinite": k["nonfinite"],
+        }
+    if args.workload in ("expand", "all"):
+        log("render-side expansion: 100k-triangle scalp, 4-point guides, 3 x 2 x 16 tessellation")
+        x = bench_expand(ba, hipm, 50, 3)
+        xgb = (x["written"] + x["read"]) / (x["ms"] / 1000.0) / 1e9
+        out["expand"] = {
+            "workload": "render-side strand expansion (the reference renderer's tessellation stream pass): "
+                        "%d-triangle scalp, %d guide strands x %d points, 3 instances x 2 lines x 16 "
+                        "subsegments" % (x["ntris"], x["strands"], x["npoints"]),
+            "vertices": x["vertices"], "bytes_written": x["written"], "bytes_read": x["read"],
+            "ms_per_pass": x["ms"], "achieved_gbs": xgb, "frac_hbm_peak": xgb / HBM_PEAK_GBS,
+            "finite": x["finite"],
         }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores)")

@@ -74,6 +74,17 @@ ftl_status ftl_scn_skinned_groom(const ftl_scn_skinned* d, const float* rest, co
    through the bone's pivot. */
 ftl_status ftl_scn_bones(const ftl_scn_skinned* d, double t, float* dq);
 
+/* Per-point tangents of strands (float4, w = 0) by central differences:
+   t_i = (p_{i+1} - p_{i-1}) / 2, one-sided at the ends.  A stand-in for the
+   reference's "curly" initial tangents (hair.cc:290-328), which need GLM's
+   simplex noise (not part of the reference tree). */
+ftl_status ftl_scn_strand_tangents(const float* pos, int nstrands, int npoints, float* tangents);
+
+/* The renderer's random table (RandomBuffer::generate_values,
+   src/memory/random_buffer.cc:22-36: uniform [0, 1) values) from a seed
+   instead of std::random_device: float4 [n], xyz uniform [0, 1), w = 0. */
+ftl_status ftl_scn_random_table(uint64_t seed, int n, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,10 @@
 //   src/fx/hair.cc:89-125                  (one update = one dispatch; the next
 //                                            step reads this step's output)
 //   src/shaders/shared/inc_skinning.glsl:22-82  (dual-quaternion blending, roots)
+//   src/shaders/hair/02_tess_stream/{vs,tcs,tes,gs}_stream_hair.glsl and
+//   src/shaders/shared/inc_maths.glsl:107-115,210-228,239-241,263-266
+//                                          (render-side strand expansion, the
+//                                           stream pass of src/fx/hair.cc:144-170)
 //
 // The GLSL leaves rounding of compound expressions to the implementation:
 // without `precise` a compiler may fuse a*b+c and apply a scalar factor once.
@@ -32,6 +36,7 @@
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off, no fast-math).
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -449,6 +454,159 @@ int oracle_max_threads(void) {
 #else
   return 1;
 #endif
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Render-side strand expansion (the tessellation stream pass).
+//
+// Patches: scalp triangle t (three strand indices), strand segment j; control
+// points P[2k], P[2k+1] = positions of points j, j+1 of the strand at vertex k
+// (hair.cc:402-411), tangents likewise, scaled by uScaleFactor
+// (tcs_stream_hair.glsl:41).  Each patch is drawn ninst times; the
+// tessellator makes nlines isolines (v = l / nlines) of nsub segments
+// (u = s / nsub, ..., (s+1) / nsub), and the geometry stage captures each
+// segment as two vertices (x, y, z, relPos) (gs_stream_hair.glsl:18-28), in
+// the order instance, triangle, segment, line, subsegment.
+
+namespace {
+
+// Index of the random entry of (instance, line): tes_stream_hair.glsl:50-53,
+// int(gl_TessCoord.y * 40 + inInstanceID) % HAIR_TF_RANDOMBUFFER_SIZE.
+int expand_random_index(int inst, int line, int nlines) {
+  const float v = (float)line / (float)nlines;
+  const float x = v * 40.0f;
+  const float y = x + (float)inst;
+  return ((int)y) % 4096;
+}
+
+// sample_triangle2's weights (inc_maths.glsl:107-115), statement by statement:
+// the second statement reads the x the first one wrote.
+template <typename T>
+void expand_weights(T rx, T ry, T c[3]) {
+  T x = rx, y = ry;
+  if (x + y > T(1)) {
+    x = std::max(x, y);
+    y = std::min(x, y);
+    x = T(1) - x;
+  }
+  c[0] = x;
+  c[1] = y;
+  c[2] = T(1) - (x + y);
+}
+
+}  // namespace
+
+extern "C" {
+
+// FP32, in the evaluation order of include/ftl_render.h (the kernel's).
+// pos, tan: float4 [nstrands*npoints]; tris: int3 [ntris]; rnd: float4
+// [nrandom]; out: float4 [2*ninst*ntris*(npoints-1)*nlines*nsub].  Returns
+// -1 when a random index or a triangle's strand index is out of range.
+int oracle_expand_f32(const float* pos, const float* tan, int nstrands, int npoints, const int* tris, int ntris,
+                      const float* rnd, int nrandom, int ninst, int nlines, int nsub, float scale, float* out) {
+  for (int i = 0; i < 3 * ntris; ++i)
+    if (tris[i] < 0 || tris[i] >= nstrands) return -1;
+  for (int inst = 0; inst < ninst; ++inst)
+    for (int line = 0; line < nlines; ++line)
+      if (expand_random_index(inst, line, nlines) >= nrandom) return -1;
+  const int nseg = npoints - 1;
+  const float n = (float)nsub, nf = (float)npoints;
+  for (int inst = 0; inst < ninst; ++inst)
+    for (int tri = 0; tri < ntris; ++tri)
+      for (int seg = 0; seg < nseg; ++seg)
+        for (int line = 0; line < nlines; ++line) {
+          const float* r = rnd + 4 * expand_random_index(inst, line, nlines);
+          float c[3];
+          expand_weights<float>(r[0], r[1], c);
+          for (int s = 0; s < nsub; ++s)
+            for (int e = 0; e < 2; ++e) {
+              const float u = (float)(s + e) / n;
+              const float u2 = u * u, u3 = u2 * u;
+              const float a = 2.0f * u3, b = 3.0f * u2;
+              const float h0 = (a - b) + 1.0f, h1 = b - a, h2 = (u3 - 2.0f * u2) + u, h3 = u3 - u2;
+              float g[3][3];
+              for (int k = 0; k < 3; ++k) {
+                const size_t base = (size_t)tris[3 * tri + k] * npoints + seg;
+                const float* p0 = pos + 4 * base;
+                const float* p1 = pos + 4 * (base + 1);
+                const float* q0 = tan + 4 * base;
+                const float* q1 = tan + 4 * (base + 1);
+                for (int d = 0; d < 3; ++d) {
+                  const float t0 = q0[d] * scale, t1 = q1[d] * scale;
+                  g[k][d] = std::fma(h3, t1, std::fma(h2, t0, std::fma(h1, p1[d], h0 * p0[d])));
+                }
+              }
+              const float rp = (float)seg / nf + u / nf;
+              const float t = std::min(std::max(rp, 0.0f), 1.0f);
+              const float w = ((t * t) * t) * (10.0f + t * (-15.0f + 6.0f * t));
+              const long long id =
+                  2 * (((((long long)inst * ntris + tri) * nseg + seg) * nlines + line) * nsub + s) + e;
+              float* o = out + 4 * id;
+              for (int d = 0; d < 3; ++d) o[d] = std::fma(c[2], g[2][d], std::fma(c[1], g[1][d], c[0] * g[0][d]));
+              o[3] = w;
+            }
+        }
+  return 0;
+}
+
+// FP64, written as the GLSL states it: hermite_mix as the row vector
+// (u^3, u^2, u, 1) times mHermit times B (inc_maths.glsl:210-228), the blend
+// as mat3x4 * coords, smoothstep2 through maprange (:239-241, :263-266).  An
+// independent evaluation of the same formulas: the accuracy reference of the
+// FP32 path.  Output in double, same layout.
+int oracle_expand_f64(const float* pos, const float* tan, int nstrands, int npoints, const int* tris, int ntris,
+                      const float* rnd, int nrandom, int ninst, int nlines, int nsub, float scale, double* out) {
+  for (int i = 0; i < 3 * ntris; ++i)
+    if (tris[i] < 0 || tris[i] >= nstrands) return -1;
+  for (int inst = 0; inst < ninst; ++inst)
+    for (int line = 0; line < nlines; ++line)
+      if (expand_random_index(inst, line, nlines) >= nrandom) return -1;
+  // mHermit, column-major as the GLSL constructor lists its columns.
+  const double Mh[4][4] = {{2, -3, 0, 1}, {-2, 3, 0, 0}, {1, -2, 1, 0}, {1, -1, 0, 0}};
+  const int nseg = npoints - 1;
+  for (int inst = 0; inst < ninst; ++inst)
+    for (int tri = 0; tri < ntris; ++tri)
+      for (int seg = 0; seg < nseg; ++seg)
+        for (int line = 0; line < nlines; ++line) {
+          const float* r = rnd + 4 * expand_random_index(inst, line, nlines);
+          double c[3];
+          expand_weights<double>(r[0], r[1], c);
+          for (int s = 0; s < nsub; ++s)
+            for (int e = 0; e < 2; ++e) {
+              const double u = (double)(s + e) / (double)nsub;
+              const double vU[4] = {u * u * u, u * u, u, 1.0};
+              double h[4];  // vU * mHermit: column i of the product = dot(vU, column i)
+              for (int i = 0; i < 4; ++i) {
+                h[i] = 0.0;
+                for (int k = 0; k < 4; ++k) h[i] += vU[k] * Mh[i][k];
+              }
+              double g[3][4];
+              for (int k = 0; k < 3; ++k) {
+                const size_t base = (size_t)tris[3 * tri + k] * npoints + seg;
+                const float* p0 = pos + 4 * base;
+                const float* p1 = pos + 4 * (base + 1);
+                const float* q0 = tan + 4 * base;
+                const float* q1 = tan + 4 * (base + 1);
+                // B's columns: (p0.d, p1.d, t0.d, t1.d) for d = x, y, z, and (1, 1, 0, 0).
+                for (int d = 0; d < 3; ++d) {
+                  const double col[4] = {p0[d], p1[d], (double)q0[d] * scale, (double)q1[d] * scale};
+                  g[k][d] = 0.0;
+                  for (int i = 0; i < 4; ++i) g[k][d] += h[i] * col[i];
+                }
+                g[k][3] = h[0] + h[1];
+              }
+              const double x = (double)seg / npoints + u / npoints;
+              const double t = std::min(std::max((x - 0.0) / (1.0 - 0.0), 0.0), 1.0);
+              const long long id =
+                  2 * (((((long long)inst * ntris + tri) * nseg + seg) * nlines + line) * nsub + s) + e;
+              double* o = out + 4 * id;
+              for (int d = 0; d < 3; ++d) o[d] = g[0][d] * c[0] + g[1][d] * c[1] + g[2][d] * c[2];
+              o[3] = t * t * t * (10.0 + t * (-15.0 + 6.0 * t));
+            }
+        }
+  return 0;
 }
 
 }  // extern "C"

@@ -61,6 +61,10 @@ def lib():
         L.oracle_skin_f64.argtypes = [C.c_int, vp, vp, vp, vp, vp]
         L.oracle_bary_roots_f32.argtypes = [C.c_int, vp, vp, vp, vp, vp]
         L.oracle_max_threads.restype = C.c_int
+        for n in ("oracle_expand_f32", "oracle_expand_f64"):
+            getattr(L, n).restype = C.c_int
+            getattr(L, n).argtypes = [vp, vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, C.c_float, vp]
         _lib = L
     return _lib
 
@@ -185,6 +189,35 @@ def bary_roots_f32(tris, tri, bary, verts):
     out = np.empty((tri.shape[0], 4), np.float32)
     lib().oracle_bary_roots_f32(tri.shape[0], _p(tris), _p(tri), _p(bary), _p(verts), _p(out))
     return out
+
+
+def expand_vertex_count(ntris, npoints, ninst=3, nlines=2, nsub=16):
+    return 2 * ninst * ntris * (npoints - 1) * nlines * nsub
+
+
+def _expand(fn, dtype, pos, tan, tris, rnd, ninst, nlines, nsub, scale):
+    pos = np.ascontiguousarray(pos, np.float32)
+    tan = np.ascontiguousarray(tan, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    rnd = np.ascontiguousarray(rnd, np.float32)
+    S, N = pos.shape[0], pos.shape[1]
+    T = tris.shape[0]
+    out = np.empty((expand_vertex_count(T, N, ninst, nlines, nsub), 4), dtype)
+    rc = fn(_p(pos), _p(tan), S, N, _p(tris), T, _p(rnd), rnd.shape[0], ninst, nlines, nsub, f32(scale), _p(out))
+    if rc != 0:
+        raise ValueError("expansion: a triangle or random index is out of range")
+    return out
+
+
+def expand_f32(pos, tan, tris, rnd, ninst=3, nlines=2, nsub=16, scale=1.0):
+    """Render-side strand expansion, FP32 (the kernel's evaluation order).
+    pos, tan: [S, N, 4]; tris: [T, 3]; rnd: [R, 4]; returns [V, 4] float32."""
+    return _expand(lib().oracle_expand_f32, np.float32, pos, tan, tris, rnd, ninst, nlines, nsub, scale)
+
+
+def expand_f64(pos, tan, tris, rnd, ninst=3, nlines=2, nsub=16, scale=1.0):
+    """The same expansion in FP64, evaluated as the GLSL writes it."""
+    return _expand(lib().oracle_expand_f64, np.float64, pos, tan, tris, rnd, ninst, nlines, nsub, scale)
 
 
 def max_threads():

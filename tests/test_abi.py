@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     names = set()
-    for h in ("ftl.h", "ftl_scenario.h", "ftl_server.h"):
+    for h in ("ftl.h", "ftl_scenario.h", "ftl_server.h", "ftl_render.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(ftl_\w+)\s*\(", text):

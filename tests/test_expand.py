@@ -1,0 +1,226 @@
+"""Render-side strand expansion (include/ftl_render.h): the stream pass of
+src/fx/hair.cc:144-170 (shaders src/shaders/hair/02_tess_stream/*).
+
+CPU tests pin the oracle (oracle_expand_f32) with what the shaders fix:
+  * an independent FP64 evaluation written as the GLSL states it (row vector
+    x mHermit x B, mat3x4 * coords, maprange + smoothstep2);
+  * a Hermite segment passes through its control points (u = 0 and 1): the
+    capture is continuous across subsegments and strand segments, bit for bit;
+  * closed forms: zero tangents give P0 + (3u^2 - 2u^3)(P1 - P0), chord
+    tangents give the straight segment; relPos is smoothstep2(j/N + u/N);
+  * which random entry each (instance, line) reads, recovered from the
+    output, is int(v*40 + instance) % 4096 (tes_stream_hair.glsl:50-53);
+  * the weights of sample_triangle2 as its statements execute (the second
+    statement reads the x the first wrote), recovered from the output;
+  * the capture order (instance, triangle, segment, line, subsegment).
+GPU tests run the kernel against the oracle bit for bit.
+"""
+import numpy as np
+import pytest
+
+from oracle import ftl_oracle as orc
+
+
+def scene(S=9, N=5, T=7, R=64, seed=0):
+    rng = np.random.default_rng(seed)
+    pos = rng.standard_normal((S, N, 4)).astype(np.float32)
+    tan = (0.3 * rng.standard_normal((S, N, 4))).astype(np.float32)
+    tris = rng.integers(0, S, size=(T, 3)).astype(np.int32)
+    rnd = rng.random((R, 4)).astype(np.float32)
+    return pos, tan, tris, rnd
+
+
+def vid(inst, tri, seg, line, s, e, T, nseg, nlines, nsub):
+    return 2 * ((((inst * T + tri) * nseg + seg) * nlines + line) * nsub + s) + e
+
+
+def test_fp32_matches_literal_fp64():
+    for seed, (ninst, nlines, nsub, scale) in enumerate([(3, 2, 16, 1.0), (1, 1, 1, 0.5), (2, 5, 7, 2.0)]):
+        pos, tan, tris, rnd = scene(seed=seed)
+        a = orc.expand_f32(pos, tan, tris, rnd, ninst, nlines, nsub, scale)
+        b = orc.expand_f64(pos, tan, tris, rnd, ninst, nlines, nsub, scale)
+        assert a.shape == (orc.expand_vertex_count(7, 5, ninst, nlines, nsub), 4)
+        err = np.abs(a.astype(np.float64) - b)
+        # |values| <= ~5 here; FP32 evaluation of a cubic and a blend
+        assert err[:, :3].max() < 4e-6 * max(1.0, np.abs(b[:, :3]).max()), err[:, :3].max()
+        assert err[:, 3].max() < 5e-7  # a few ulp of values in [0, 1]
+
+
+def test_continuity_through_control_points():
+    pos, tan, tris, rnd = scene(S=6, N=6, T=4, seed=3)
+    ninst, nlines, nsub = 2, 3, 5
+    out = orc.expand_f32(pos, tan, tris, rnd, ninst, nlines, nsub)
+    T, nseg = 4, 5
+    for inst in range(ninst):
+        for tri in range(T):
+            for line in range(nlines):
+                for seg in range(nseg):
+                    for s in range(nsub - 1):
+                        a = out[vid(inst, tri, seg, line, s, 1, T, nseg, nlines, nsub)]
+                        b = out[vid(inst, tri, seg, line, s + 1, 0, T, nseg, nlines, nsub)]
+                        assert a.tobytes() == b.tobytes()
+                    if seg + 1 < nseg:  # end of segment j = start of segment j+1 (the shared control point)
+                        a = out[vid(inst, tri, seg, line, nsub - 1, 1, T, nseg, nlines, nsub)]
+                        b = out[vid(inst, tri, seg + 1, line, 0, 0, T, nseg, nlines, nsub)]
+                        assert a[:3].tobytes() == b[:3].tobytes()
+
+
+def _single_guide_scene(pos, tan):
+    """All three triangle vertices on strand 0: the blend weights sum to one."""
+    tris = np.zeros((1, 3), np.int32)
+    rnd = np.full((64, 4), 0.25, np.float32)
+    return tris, rnd
+
+
+def test_closed_forms():
+    N, nsub = 3, 8
+    P = np.array([[0.0, 0.0, 0.0], [1.0, 2.0, -1.0], [1.5, 2.5, 0.5]], np.float32)
+    pos = np.zeros((1, N, 4), np.float32)
+    pos[0, :, :3] = P
+    tris, rnd = _single_guide_scene(pos, None)
+    u = np.arange(nsub + 1) / nsub
+    # zero tangents: P0 + (3u^2 - 2u^3)(P1 - P0)
+    tan = np.zeros_like(pos)
+    out = orc.expand_f32(pos, tan, tris, rnd, 1, 1, nsub).astype(np.float64)
+    for seg in range(N - 1):
+        got = np.concatenate([out[2 * (seg * nsub + s)][None, :3] for s in range(nsub)]
+                             + [out[2 * (seg * nsub + nsub - 1) + 1][None, :3]])
+        want = P[seg] + np.outer(3 * u ** 2 - 2 * u ** 3, P[seg + 1] - P[seg])
+        assert np.abs(got - want).max() < 2e-6
+    # tangent = chord at both ends: the straight segment P0 + u (P1 - P0)
+    tan = np.zeros_like(pos)
+    for seg in range(N - 1):
+        pos2 = pos.copy()
+        tan2 = np.zeros_like(pos)
+        tan2[0, seg, :3] = P[seg + 1] - P[seg]
+        tan2[0, seg + 1, :3] = P[seg + 1] - P[seg]
+        out = orc.expand_f32(pos2, tan2, tris, rnd, 1, 1, nsub).astype(np.float64)
+        got = np.array([out[2 * (seg * nsub + s)][:3] for s in range(nsub)])
+        want = P[seg] + np.outer(u[:-1], P[seg + 1] - P[seg])
+        assert np.abs(got - want).max() < 2e-6
+    # relPos = 6x^5 - 15x^4 + 10x^3 with x = j/N + u/N
+    for seg in range(N - 1):
+        for s in range(nsub):
+            x = seg / N + (s / nsub) / N
+            assert abs(out[2 * (seg * nsub + s)][3] - (6 * x ** 5 - 15 * x ** 4 + 10 * x ** 3)) < 3e-7
+
+
+def _corner_scene(rnd, ninst, nlines):
+    """Three straight strands at the corners A, B, C of a triangle in z = 0 with
+    zero tangents: the vertex at u = 0 is c0 A + c1 B + c2 C."""
+    N = 2
+    A, B, Cc = np.array([0.0, 0, 0]), np.array([1.0, 0, 0]), np.array([0.0, 1, 0])
+    pos = np.zeros((3, N, 4), np.float32)
+    for k, X in enumerate((A, B, Cc)):
+        pos[k, 0, :3] = X
+        pos[k, 1, :3] = X + np.array([0, 0, 1.0])
+    tan = np.zeros_like(pos)
+    tris = np.array([[0, 1, 2]], np.int32)
+    out = orc.expand_f32(pos, tan, tris, rnd, ninst, nlines, 1)
+    w = {}
+    for inst in range(ninst):
+        for line in range(nlines):
+            v = out[vid(inst, 0, 0, line, 0, 0, 1, 1, nlines, 1)].astype(np.float64)
+            c1, c2 = v[0], v[1]  # weights of B (x) and C (y); A's is the rest
+            w[(inst, line)] = (1.0 - c1 - c2, c1, c2)
+    return w
+
+
+def test_random_index_per_instance_and_line():
+    ninst, nlines = 3, 2
+    rnd = np.zeros((64, 4), np.float32)
+    rnd[:, 0] = np.arange(64) / 200.0   # x + y <= 1: no folding
+    rnd[:, 1] = 0.5
+    w = _corner_scene(rnd, ninst, nlines)
+    for (inst, line), c in w.items():   # c0 = the entry's x
+        got = int(round(c[0] * 200.0))
+        assert got == int(np.float32(np.float32(line) / np.float32(nlines)) * 40 + inst) % 4096, (inst, line, got)
+    assert sorted(int(round(c[0] * 200)) for c in w.values()) == [0, 1, 2, 20, 21, 22]
+
+
+def test_sample_triangle2_statement_order():
+    # (x, y) with x + y > 1: the shader's statements give x' = 1 - max(x, y), y' = y
+    cases = [(0.9, 0.6), (0.6, 0.9), (0.7, 0.7), (0.3, 0.4)]
+    rnd = np.zeros((64, 4), np.float32)
+    for i, (x, y) in enumerate(cases):
+        rnd[i, :2] = (x, y)
+    w = _corner_scene(rnd, 4, 1)   # line 0: entry = instance
+    for i, (x, y) in enumerate(cases):
+        xf, yf = np.float32(x), np.float32(y)
+        if xf + yf > 1:
+            want = (np.float32(1) - max(xf, yf), yf)
+        else:
+            want = (xf, yf)
+        got = w[(i, 0)]
+        assert abs(got[0] - want[0]) < 1e-6 and abs(got[1] - want[1]) < 1e-6, (x, y, got, want)
+        assert min(got) > -1e-6 and abs(sum(got) - 1.0) < 1e-6
+
+
+def test_errors_and_sizes():
+    pos, tan, tris, rnd = scene()
+    bad = tris.copy()
+    bad[0, 0] = pos.shape[0]
+    with pytest.raises(ValueError):
+        orc.expand_f32(pos, tan, bad, rnd)
+    with pytest.raises(ValueError):   # line 1 of 2 reads entry 20 + instance
+        orc.expand_f32(pos, tan, tris, rnd[:20])
+    assert orc.expand_vertex_count(100, 4) == 2 * 3 * 100 * 3 * 2 * 16
+
+
+def test_generators():
+    import barbu_amd as ba
+    r = ba.random_table(7, 4096)
+    assert r.shape == (4096, 4) and (r[:, :3] >= 0).all() and (r[:, :3] < 1).all() and (r[:, 3] == 0).all()
+    assert np.array_equal(r, ba.random_table(7, 4096)) and not np.array_equal(r, ba.random_table(8, 4096))
+    assert abs(r[:, :3].mean() - 0.5) < 0.02
+    pos, _, _ = ba.sphere_groom(5, 6, seed=2)
+    t = ba.strand_tangents(pos)
+    assert np.allclose(t[:, 1:-1, :3], 0.5 * (pos[:, 2:, :3] - pos[:, :-2, :3]), atol=1e-7)
+    assert np.allclose(t[:, 0, :3], pos[:, 1, :3] - pos[:, 0, :3], atol=1e-7)
+    assert np.allclose(t[:, -1, :3], pos[:, -1, :3] - pos[:, -2, :3], atol=1e-7)
+
+
+# --- GPU ---------------------------------------------------------------------
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("S,N,T,ninst,nlines,nsub,scale", [
+    (9, 5, 7, 3, 2, 16, 1.0), (40, 2, 70, 1, 1, 1, 0.7), (33, 4, 65, 2, 3, 5, 1.3), (300, 32, 500, 3, 2, 16, 1.0)])
+def test_gpu_matches_oracle(ctx, S, N, T, ninst, nlines, nsub, scale):
+    import barbu_amd as ba
+    print("expand S=%d N=%d T=%d inst=%d lines=%d sub=%d" % (S, N, T, ninst, nlines, nsub), flush=True)
+    pos, tan, tris, rnd = scene(S, N, T, R=4096, seed=S)
+    g = ba.Groom(ctx, pos)
+    e = ba.Expand(ctx, S, N, tris, tan, rnd)
+    p = ba.expand_params(ninst, nlines, nsub, scale)
+    got = e.run_groom(g, p)
+    want = orc.expand_f32(pos, tan, tris, rnd, ninst, nlines, nsub, scale)
+    from tests.helpers import assert_bitexact
+    assert_bitexact(got, want, "expansion")
+    e.close()
+    g.close()
+
+
+@pytest.mark.gpu
+def test_gpu_errors(ctx):
+    import barbu_amd as ba
+    pos, tan, tris, rnd = scene(R=64)
+    bad = tris.copy()
+    bad[2, 1] = -1
+    with pytest.raises(ba.FtlError):
+        ba.Expand(ctx, pos.shape[0], pos.shape[1], bad, tan, rnd)
+    g = ba.Groom(ctx, pos)
+    e = ba.Expand(ctx, pos.shape[0], pos.shape[1], tris, tan, rnd[:10])
+    with pytest.raises(ba.FtlError):       # line 1 reads entry 20
+        e.run_groom(g)
+    e.close()
+    e = ba.Expand(ctx, pos.shape[0], pos.shape[1], tris, tan, rnd)
+    n = e.vertex_count()
+    buf = ba.DeviceBuffer(ctx, n * 16)
+    p, _ = g.device_buffers()
+    with pytest.raises(ba.FtlError):       # one vertex short
+        e.run(p, buf.ptr, n - 1)
+    with pytest.raises(ba.FtlError):
+        e.run(p, buf.ptr, n, ba.expand_params(nsubsegments=65))
+    buf.close()
+    e.close()
+    g.close()
