@@ -102,7 +102,11 @@ class Context:
 class Groom:
     """One groom's device state (ftl_groom_*)."""
 
-    def __init__(self, ctx, pos, vel=None, rest_roots=None):
+    def __init__(self, ctx, pos, vel=None, rest_roots=None, _handle=None):
+        if _handle is not None:   # an existing handle (device-side initialisation)
+            self.ctx, self._h = ctx, _handle
+            self.nstrands, self.npoints = int(pos[0]), int(pos[1])
+            return
         pos = _f32(pos)
         n = pos.shape[0]
         self.nstrands, self.npoints = int(pos.shape[0]), int(pos.shape[1])
@@ -462,6 +466,18 @@ class Expand:
         tangents = _f32(tangents).reshape(self.nstrands * self.npoints, 4)
         check(lib().ftl_expand_set_tangents(self._h, _ptr(tangents)), "ftl_expand_set_tangents")
 
+    def reference_tangents(self, normals, maxlength, curly=True):
+        """The reference's initial tangents, computed on the device (ftl_expand_reference_tangents)."""
+        normals = _f32(normals).reshape(self.nstrands, 3)
+        check(lib().ftl_expand_reference_tangents(self._h, _ptr(normals), float(maxlength), 1 if curly else 0),
+              "ftl_expand_reference_tangents")
+
+    def tangents(self):
+        """The current tangents, float32 [nstrands, npoints, 4] (ftl_expand_get_tangents)."""
+        out = np.empty((self.nstrands, self.npoints, 4), np.float32)
+        check(lib().ftl_expand_get_tangents(self._h, _ptr(out)), "ftl_expand_get_tangents")
+        return out
+
     def vertex_count(self, params=None):
         p = params or expand_params()
         return int(lib().ftl_expand_vertex_count(self._h, C.byref(p)))
@@ -524,6 +540,16 @@ def sphere_groom(nstrands, npoints, center=(0, 0, 0), radius=1.0, length=0.2, ji
     roots = np.empty((nstrands, 4), np.float32)
     check(lib().ftl_scn_sphere_groom(C.byref(d), _ptr(pos), _ptr(vel), _ptr(roots)), "ftl_scn_sphere_groom")
     return pos, vel, roots
+
+
+def reference_groom_device(ctx, roots, normals, npoints=4, maxlength=0.5, seed=0):
+    """Hair::init_simulation's state built on the device (ftl_groom_create_reference): a Groom."""
+    roots = _f32(roots).reshape(-1, 3)
+    normals = _f32(normals).reshape(-1, 3)
+    h = C.c_void_p()
+    check(lib().ftl_groom_create_reference(ctx.handle, roots.shape[0], _ptr(roots), _ptr(normals), int(npoints),
+                                           float(maxlength), int(seed), C.byref(h)), "ftl_groom_create_reference")
+    return Groom(ctx, (roots.shape[0], int(npoints)), _handle=h)
 
 
 def reference_groom(roots, normals, npoints=4, maxlength=0.5, seed=0):

@@ -207,8 +207,11 @@ SIGNATURES = {
     "ftl_expand_create": (st, [vp, C.c_int, C.c_int, vp, C.c_int, vp, vp, C.c_int, pp]),
     "ftl_expand_destroy": (None, [vp]),
     "ftl_expand_set_tangents": (st, [vp, vp]),
+    "ftl_expand_get_tangents": (st, [vp, vp]),
     "ftl_expand_vertex_count": (C.c_int64, [vp, C.POINTER(ExpandParams)]),
     "ftl_expand_run": (st, [vp, C.POINTER(ExpandParams), vp, vp, C.c_int64, vp]),
+    "ftl_expand_reference_tangents": (st, [vp, vp, C.c_float, C.c_int]),
+    "ftl_groom_create_reference": (st, [vp, C.c_int, vp, vp, C.c_int, C.c_float, C.c_uint64, pp]),
 }
 
 _lib = None

@@ -228,11 +228,18 @@ static bool is_identity(const float m[16]) {
 
 extern "C" ftl_status ftl_groom_create(ftl_ctx* ctx, const ftl_groom_desc* d, ftl_groom** out) {
   if (!ctx || !d || !out) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!d->positions) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "positions is NULL");
+  return ftl_groom_create_uninitialised(ctx, d, out);
+}
+
+// The groom of d; its state is copied from d->positions / d->velocities when
+// given, zero otherwise (device-side initialisation fills it afterwards).
+ftl_status ftl_groom_create_uninitialised(ftl_ctx* ctx, const ftl_groom_desc* d, ftl_groom** out) {
+  if (!ctx || !d || !out) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL argument");
   *out = nullptr;
   if (d->nstrands <= 0) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "nstrands must be > 0 (got %d)", d->nstrands);
   if (d->npoints < 2 || d->npoints > FTL_MAX_POINTS)
     return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "npoints must be in [2, %d] (got %d)", FTL_MAX_POINTS, d->npoints);
-  if (!d->positions) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "positions is NULL");
   const size_t n = (size_t)d->nstrands * (size_t)d->npoints;
   if (n > (size_t)INT32_MAX * 4) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom too large");
   hipError_t e;
@@ -259,7 +266,8 @@ extern "C" ftl_status ftl_groom_create(ftl_ctx* ctx, const ftl_groom_desc* d, ft
     delete g;
     return st;
   }
-  bool ok = hipMemcpy(g->d_pos, d->positions, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  bool ok = d->positions ? hipMemcpy(g->d_pos, d->positions, bytes, hipMemcpyHostToDevice) == hipSuccess
+                         : hipMemset(g->d_pos, 0, bytes) == hipSuccess;
   if (d->velocities)
     ok = ok && hipMemcpy(g->d_vel, d->velocities, bytes, hipMemcpyHostToDevice) == hipSuccess;
   else

@@ -61,7 +61,20 @@ struct ftl_groom {
   int lanes = 0;
 };
 
+// Render-side expansion object (ftl_render.hip); its tangents are also
+// written by the device-side initialisation (ftl_init.hip).
+struct ftl_expand {
+  ftl_ctx* ctx = nullptr;
+  int nstrands = 0, npoints = 0, ntris = 0, nrandom = 0;
+  int4* d_tris = nullptr;
+  float4* d_tan = nullptr;
+  float4* d_rand = nullptr;
+};
+
 ftl_status ftl_fail(ftl_status st, const char* fmt, ...);
+// ftl_groom_create without a host state: zero state when desc->positions is
+// NULL (filled on the device afterwards).
+ftl_status ftl_groom_create_uninitialised(ftl_ctx* ctx, const ftl_groom_desc* d, ftl_groom** out);
 ftl_status ftl_hip_fail(hipError_t e, const char* what);
 ftl_status ftl_dev_alloc(ftl_ctx* ctx, void** p, size_t bytes, const char* what);
 void ftl_dev_free(ftl_ctx* ctx, void* p, size_t bytes);

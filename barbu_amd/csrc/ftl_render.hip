@@ -18,14 +18,6 @@
 #include "ftl_internal.h"
 #include "ftl_render.h"
 
-struct ftl_expand {
-  ftl_ctx* ctx = nullptr;
-  int nstrands = 0, npoints = 0, ntris = 0, nrandom = 0;
-  int4* d_tris = nullptr;
-  float4* d_tan = nullptr;
-  float4* d_rand = nullptr;
-};
-
 namespace {
 
 using ftl_dev::f3;
@@ -257,6 +249,17 @@ extern "C" ftl_status ftl_expand_set_tangents(ftl_expand* e, const float* tangen
   if ((he = hipMemcpy(e->d_tan, tangents, (size_t)e->nstrands * e->npoints * sizeof(float4), hipMemcpyHostToDevice)) !=
       hipSuccess)
     return ftl_hip_fail(he, "ftl_expand_set_tangents");
+  return FTL_OK;
+}
+
+extern "C" ftl_status ftl_expand_get_tangents(ftl_expand* e, float* tangents) {
+  if (!e || !tangents) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  hipError_t he;
+  if ((he = hipSetDevice(e->ctx->device)) != hipSuccess) return ftl_hip_fail(he, "hipSetDevice");
+  if ((he = hipDeviceSynchronize()) != hipSuccess) return ftl_hip_fail(he, "ftl_expand_get_tangents");
+  if ((he = hipMemcpy(tangents, e->d_tan, (size_t)e->nstrands * e->npoints * sizeof(float4), hipMemcpyDeviceToHost)) !=
+      hipSuccess)
+    return ftl_hip_fail(he, "ftl_expand_get_tangents");
   return FTL_OK;
 }
 

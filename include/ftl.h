@@ -133,6 +133,15 @@ ftl_status ftl_ctx_memory(ftl_ctx* ctx, uint64_t* bytes);
    ftl_params_reference, root mode FTL_ROOT_MODEL with M = identity, and one
    collider: the shader's default uBoundingSphere (0,0,0,1) (:43). */
 ftl_status ftl_groom_create(ftl_ctx* ctx, const ftl_groom_desc* desc, ftl_groom** out);
+/* ≙ Hair::setup + init_simulation's positions and velocities
+   (hair.cc:236-288), computed on the device: strand j rooted at roots[j]
+   (float3) along normals[j] (float3), offset_i = ((float)i * (maxlength /
+   npoints)) * r_j with the per-strand factor r_j = (float)(1 + 0.1 (1 - 2 U_j))
+   from the seed's SplitMix64 stream in place of rand(); w = offset_i -
+   offset_{i-1}; zero velocities.  Bit-identical to ftl_scn_reference_groom
+   (ftl_scenario.h), without the host-side state or its upload. */
+ftl_status ftl_groom_create_reference(ftl_ctx* ctx, int nroots, const float* roots, const float* normals,
+                                      int npoints, float maxlength, uint64_t seed, ftl_groom** out);
 /* ≙ Hair::deinit (hair.cc:66-87).  NULL is ignored. */
 void ftl_groom_destroy(ftl_groom* g);
 

@@ -58,6 +58,20 @@ ftl_status ftl_expand_create(ftl_ctx* ctx, int nstrands, int npoints, const int*
 void ftl_expand_destroy(ftl_expand* e);
 /* Replaces the tangents (host float4 [nstrands*npoints]). */
 ftl_status ftl_expand_set_tangents(ftl_expand* e, const float* tangents);
+/* Copies the tangents to the host (float4 [nstrands*npoints]); waits for the device. */
+ftl_status ftl_expand_get_tangents(ftl_expand* e, float* tangents);
+
+/* The reference's initial tangents (Hair::init_simulation, hair.cc:290-328),
+   computed on the device into e's tangents: normals = float3 per strand (the
+   roots' normals), maxlength = params_.sim.maxlength.  Per strand j (dj =
+   (j + 1) / nstrands): the root gets 0.15 n; the tip 0.2 (-n + c) with, when
+   curly, c = (cos(4 pi k), -0.71 k, sin(2.7 pi k)), k = 1.25 simplex(sin 3dj,
+   cos 5); inner point i gets s_i c_i with s_i = ((0.1 i) / (N-1)) * 0.125
+   sqrt(maxlength) and, when curly, c_i = -5.8 (10.7 cos(pi k_i), -2.3 k_i,
+   20.5 sin(pi k_i)), k_i = d_i simplex(sin 43dj, cos 5d_i), d_i = 10 (N-1-i) /
+   (N-2) (0 otherwise).  simplex is GLM's 2-D simplex noise (glm/gtc/noise),
+   restated from its published algorithm. */
+ftl_status ftl_expand_reference_tangents(ftl_expand* e, const float* normals, float maxlength, int curly);
 
 /* Vertices one run writes: 2 * ninstances * ntris * (npoints-1) * nlines *
    nsubsegments (two per line segment); -1 on invalid parameters. */

@@ -22,6 +22,9 @@
 //   src/shaders/shared/inc_maths.glsl:107-115,210-228,239-241,263-266
 //                                          (render-side strand expansion, the
 //                                           stream pass of src/fx/hair.cc:144-170)
+//   src/fx/hair.cc:236-328                 (init_simulation: positions, velocities,
+//                                           "curly" tangents; glm::simplex restated
+//                                           from the published webgl-noise algorithm)
 //
 // The GLSL leaves rounding of compound expressions to the implementation:
 // without `precise` a compiler may fuse a*b+c and apply a scalar factor once.
@@ -607,6 +610,135 @@ int oracle_expand_f64(const float* pos, const float* tan, int nstrands, int npoi
             }
         }
   return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Groom initialisation (Hair::init_simulation, hair.cc:236-328).
+//
+// rand() is replaced by SplitMix64 (Steele, Lea, Flood 2014): the j-th draw of
+// the stream s_{k+1} = s_k + 0x9e3779b97f4a7c15 (s_0 = seed), mixed by two
+// xor-shift-multiply rounds, as a double in [0, 1) from its top 53 bits.
+
+namespace {
+
+uint64_t init_draw(uint64_t seed, uint64_t j) {
+  uint64_t s = seed;
+  uint64_t z = 0;
+  for (uint64_t k = 0; k <= j; ++k) {  // sequential on purpose: the stream, not a jump
+    s += 0x9e3779b97f4a7c15ull;
+    z = s;
+  }
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// glm::simplex(vec2) (glm/gtc/noise.inl; the GLM tree is not part of the
+// reference), from the published algorithm: McEwan, Sheets, Gustavson,
+// Richardson, "Efficient computational noise in GLSL", JGT 17(2), 2012
+// (webgl-noise, snoise(vec2)), in single precision, operations in the order
+// the GLSL writes them.
+float noise_mod289(float x) { return x - std::floor(x * (1.0f / 289.0f)) * 289.0f; }
+float noise_permute(float x) { return noise_mod289(((x * 34.0f) + 1.0f) * x); }
+
+float simplex2_f32(float vx, float vy) {
+  const float C[4] = {0.211324865405187f, 0.366025403784439f, -0.577350269189626f, 0.024390243902439f};
+  const float skew = vx * C[1] + vy * C[1];
+  float i[2] = {std::floor(vx + skew), std::floor(vy + skew)};
+  const float unskew = i[0] * C[0] + i[1] * C[0];
+  const float x0[2] = {vx - i[0] + unskew, vy - i[1] + unskew};
+  const float i1[2] = {x0[0] > x0[1] ? 1.0f : 0.0f, x0[0] > x0[1] ? 0.0f : 1.0f};
+  const float x12[4] = {(x0[0] + C[0]) - i1[0], (x0[1] + C[0]) - i1[1], x0[0] + C[2], x0[1] + C[2]};
+  for (int k = 0; k < 2; ++k) i[k] = i[k] - 289.0f * std::floor(i[k] / 289.0f);
+  const float oy[3] = {0.0f, i1[1], 1.0f}, ox[3] = {0.0f, i1[0], 1.0f};
+  float p[3], m[3];
+  for (int k = 0; k < 3; ++k) p[k] = noise_permute(noise_permute(i[1] + oy[k]) + i[0] + ox[k]);
+  const float d[3] = {x0[0] * x0[0] + x0[1] * x0[1], x12[0] * x12[0] + x12[1] * x12[1],
+                      x12[2] * x12[2] + x12[3] * x12[3]};
+  for (int k = 0; k < 3; ++k) {
+    m[k] = std::max(0.5f - d[k], 0.0f);
+    m[k] = m[k] * m[k];
+  }
+  for (int k = 0; k < 3; ++k) m[k] = m[k] * m[k];
+  float g[3];
+  const float gx[3] = {x0[0], x12[0], x12[2]}, gy[3] = {x0[1], x12[1], x12[3]};
+  for (int k = 0; k < 3; ++k) {
+    const float fr = p[k] * C[3];
+    const float x = 2.0f * (fr - std::floor(fr)) - 1.0f;
+    const float h = std::fabs(x) - 0.5f;
+    const float a0 = x - std::floor(x + 0.5f);
+    m[k] *= 1.79284291400159f - 0.85373472095314f * (a0 * a0 + h * h);
+    g[k] = a0 * gx[k] + h * gy[k];
+  }
+  return 130.0f * (m[0] * g[0] + m[1] * g[1] + m[2] * g[2]);
+}
+
+}  // namespace
+
+extern "C" {
+
+// roots, normals: float3 [nroots]; pos, vel: float4 [nroots*npoints].
+void oracle_reference_groom_f32(int nroots, const float* roots, const float* normals, int npoints, float maxlength,
+                                uint64_t seed, float* pos, float* vel) {
+  const float scaleOffset = maxlength / static_cast<float>(npoints);
+  for (int j = 0; j < nroots; ++j) {
+    const double u = (double)(init_draw(seed, (uint64_t)j) >> 11) * 0x1p-53;
+    const float r = static_cast<float>(1.0 + 0.1 * (1.0 - 2.0 * u));
+    float last = 0.0f;
+    for (int i = 0; i < npoints; ++i) {
+      const float offset = static_cast<float>(i) * scaleOffset * r;
+      float* P = pos + 4 * ((size_t)j * npoints + i);
+      for (int c = 0; c < 3; ++c) P[c] = roots[3 * j + c] + offset * normals[3 * j + c];
+      P[3] = offset - last;
+      float* V = vel + 4 * ((size_t)j * npoints + i);
+      V[0] = V[1] = V[2] = V[3] = 0.0f;
+      last = offset;
+    }
+  }
+}
+
+float oracle_simplex2_f32(float x, float y) { return simplex2_f32(x, y); }
+
+// The tangents of hair.cc:290-328 (curly or not), float4 [nstrands*npoints].
+void oracle_reference_tangents_f32(int nstrands, int npoints, const float* normals, float maxlength, int curly,
+                                   float* tan) {
+  const float inv_nroots = 1.0f / static_cast<float>(nstrands);
+  const float kPi = 3.14159265358979323846f;
+  const float scaleMaxLength = 0.125f * std::sqrt(maxlength);
+  for (int j = 0; j < nstrands; ++j) {
+    const int A = j * npoints, B = A + (npoints - 1);
+    const float dj = static_cast<float>(j + 1) * inv_nroots;
+    const float* n = normals + 3 * j;
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    if (curly) {
+      const float k = 1.25f * simplex2_f32(std::sin(3.0f * dj), std::cos(5.0f));
+      c[0] = std::cos(k * 4.0f * kPi);
+      c[1] = -0.71f * k;
+      c[2] = std::sin(k * 2.7f * kPi);
+    }
+    for (int d = 0; d < 3; ++d) {
+      tan[4 * (size_t)A + d] = 0.15f * n[d];
+      tan[4 * (size_t)B + d] = 0.2f * (-n[d] + c[d]);
+    }
+    tan[4 * (size_t)A + 3] = 0.0f;
+    tan[4 * (size_t)B + 3] = 0.0f;
+    const float dist = static_cast<float>(B - A);
+    const float inv_dist = 1.0f / dist;
+    for (int i = A + 1; i < B; ++i) {
+      if (curly) {
+        const float di = 10.0f * static_cast<float>(B - i) / (dist - 1.0f);
+        const float k = di * simplex2_f32(std::sin(43.0f * dj), std::cos(5.0f * di));
+        c[0] = -5.8f * (10.7f * std::cos(k * kPi));
+        c[1] = -5.8f * (-2.3f * k);
+        c[2] = -5.8f * (20.5f * std::sin(k * kPi));
+      }
+      const float s = 0.1f * static_cast<float>(i - A) * inv_dist * scaleMaxLength;
+      for (int d = 0; d < 3; ++d) tan[4 * (size_t)i + d] = s * c[d];
+      tan[4 * (size_t)i + 3] = s * 0.0f;
+    }
+  }
 }
 
 }  // extern "C"

@@ -65,6 +65,10 @@ def lib():
             getattr(L, n).restype = C.c_int
             getattr(L, n).argtypes = [vp, vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, C.c_int,
                                       C.c_int, C.c_float, vp]
+        L.oracle_reference_groom_f32.argtypes = [C.c_int, vp, vp, C.c_int, C.c_float, C.c_uint64, vp, vp]
+        L.oracle_simplex2_f32.restype = C.c_float
+        L.oracle_simplex2_f32.argtypes = [C.c_float, C.c_float]
+        L.oracle_reference_tangents_f32.argtypes = [C.c_int, C.c_int, vp, C.c_float, C.c_int, vp]
         _lib = L
     return _lib
 
@@ -218,6 +222,29 @@ def expand_f32(pos, tan, tris, rnd, ninst=3, nlines=2, nsub=16, scale=1.0):
 def expand_f64(pos, tan, tris, rnd, ninst=3, nlines=2, nsub=16, scale=1.0):
     """The same expansion in FP64, evaluated as the GLSL writes it."""
     return _expand(lib().oracle_expand_f64, np.float64, pos, tan, tris, rnd, ninst, nlines, nsub, scale)
+
+
+def reference_groom_f32(roots, normals, npoints, maxlength, seed):
+    """Hair::init_simulation's positions/velocities (seeded SplitMix64 draws)."""
+    roots = np.ascontiguousarray(roots, np.float32).reshape(-1, 3)
+    normals = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    n = roots.shape[0]
+    pos = np.empty((n, npoints, 4), np.float32)
+    vel = np.empty_like(pos)
+    lib().oracle_reference_groom_f32(n, _p(roots), _p(normals), npoints, f32(maxlength), int(seed), _p(pos), _p(vel))
+    return pos, vel
+
+
+def simplex2_f32(x, y):
+    return float(lib().oracle_simplex2_f32(f32(x), f32(y)))
+
+
+def reference_tangents_f32(normals, npoints, maxlength, curly=True):
+    normals = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    n = normals.shape[0]
+    out = np.empty((n, npoints, 4), np.float32)
+    lib().oracle_reference_tangents_f32(n, npoints, _p(normals), f32(maxlength), 1 if curly else 0, _p(out))
+    return out
 
 
 def max_threads():
