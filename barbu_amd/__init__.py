@@ -512,6 +512,32 @@ class Expand:
             pass
 
 
+def marschner_params(**kw):
+    """Marschner shading parameters (reference defaults, then the given fields)."""
+    p = _ffi.MarschnerParams()
+    lib().ftl_marschner_params_reference(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, float(v))
+    return p
+
+
+def marschner_luts(ctx, params=None, res=128, stream=None):
+    """Marschner::generate on the device (ftl_marschner_luts): (M, N), float32 [res, res, 4] each."""
+    from . import _hip
+    p = params or marschner_params()
+    nb = res * res * 16
+    m, n = DeviceBuffer(ctx, nb), DeviceBuffer(ctx, nb)
+    try:
+        check(lib().ftl_marschner_luts(ctx.handle, C.byref(p), int(res), m.ptr, n.ptr, stream), "ftl_marschner_luts")
+        _hip.device_synchronize()
+        return m.download(np.float32, (res, res, 4)), n.download(np.float32, (res, res, 4))
+    finally:
+        m.close()
+        n.close()
+
+
 # --- scenario generators (CPU) ----------------------------------------------
 
 def strand_tangents(pos):

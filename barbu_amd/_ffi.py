@@ -50,6 +50,11 @@ class ExpandParams(C.Structure):
     ]
 
 
+class MarschnerParams(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("eta", "absorption", "eccentricity", "ar", "br", "glint_scale",
+                                          "azimuthal_width", "delta_caustic", "delta_hm")]
+
+
 class GroomDesc(C.Structure):
     _fields_ = [
         ("nstrands", C.c_int),
@@ -211,6 +216,8 @@ SIGNATURES = {
     "ftl_expand_vertex_count": (C.c_int64, [vp, C.POINTER(ExpandParams)]),
     "ftl_expand_run": (st, [vp, C.POINTER(ExpandParams), vp, vp, C.c_int64, vp]),
     "ftl_expand_reference_tangents": (st, [vp, vp, C.c_float, C.c_int]),
+    "ftl_marschner_params_reference": (None, [C.POINTER(MarschnerParams)]),
+    "ftl_marschner_luts": (st, [vp, C.POINTER(MarschnerParams), C.c_int, vp, vp, vp]),
     "ftl_groom_create_reference": (st, [vp, C.c_int, vp, vp, C.c_int, C.c_float, C.c_uint64, pp]),
 }
 

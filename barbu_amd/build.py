@@ -36,7 +36,7 @@ def _units():
     units = []
     for k in range(NCLASSES):
         units.append(("ftl_plain.hip", ["-DFTL_CLASS_ID=%d" % k], "plain_%d.o" % k))
-    for src in ("ftl_misc.hip", "ftl_render.hip", "ftl_init.hip", "ftl_api.cpp", "ftl_batch_api.cpp", "ftl_autotune.cpp",
+    for src in ("ftl_misc.hip", "ftl_render.hip", "ftl_init.hip", "ftl_marschner.hip", "ftl_api.cpp", "ftl_batch_api.cpp", "ftl_autotune.cpp",
                 "ftl_scenario.cpp", "ftl_server.cpp"):
         units.append((src, [], os.path.splitext(src)[0] + ".o"))
     return units

@@ -492,6 +492,33 @@ def bench_expand(ba, hipm, reps, warmup, ntris=100_000, npoints=4, seed=5):
                 finite=finite)
 
 
+def bench_marschner(ba, hipm, reps=200, res=128):
+    """Marschner::generate (the two shading lookup tables) on the device."""
+    ctx = context(ba)
+    p = ba.marschner_params()
+    nb = res * res * 16
+    m, n = ba.DeviceBuffer(ctx, nb), ba.DeviceBuffer(ctx, nb)
+    stream = hipm.Stream()
+    e0, e1 = hipm.Event(), hipm.Event()
+    from barbu_amd import _ffi
+    import ctypes as C
+    run = lambda: _ffi.check(_ffi.lib().ftl_marschner_luts(ctx.handle, C.byref(p), res, m.ptr, n.ptr, stream.h),
+                             "ftl_marschner_luts")
+    for _ in range(5):
+        run()
+    hipm.device_synchronize()
+    e0.record(stream.h)
+    for _ in range(reps):
+        run()
+    e1.record(stream.h)
+    hipm.device_synchronize()
+    us = e0.elapsed_ms(e1) * 1000.0 / reps
+    m.close()
+    n.close()
+    ctx.close()
+    return dict(res=res, us=us)
+
+
 def load_traffic(workload):
     """HBM bytes per launch of the workload's kernel, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE runs summarised in profiles/traffic.json (None if
@@ -513,7 +540,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--workload", default="all",
-                    choices=["single", "batched", "served", "skinned", "expand", "all"])
+                    choices=["single", "batched", "served", "skinned", "expand", "marschner", "all"])
     ap.add_argument("--served-seconds", type=float, default=4.0)
     ap.add_argument("--skinned-steps", type=int, default=120)
     ap.add_argument("--batched-frames", type=int, default=60)
@@ -618,6 +645,12 @@ def main():
             "ms_per_pass": x["ms"], "achieved_gbs": xgb, "frac_hbm_peak": xgb / HBM_PEAK_GBS,
             "finite": x["finite"],
         }
+    if args.workload in ("marschner", "all"):
+        log("Marschner lookup tables: 2 x 128 x 128")
+        mr = bench_marschner(ba, hipm)
+        out["marschner"] = {"workload": "Marschner::generate: M and N lookup tables, 128 x 128 float4 each, "
+                                        "reference shading parameters",
+                            "us_per_generate": mr["us"]}
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores)")
         out["cpu_baseline"] = cpu_baseline(ba, args.cpu_seconds)

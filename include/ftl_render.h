@@ -96,6 +96,36 @@ int64_t ftl_expand_vertex_count(const ftl_expand* e, const ftl_expand_params* p)
 ftl_status ftl_expand_run(ftl_expand* e, const ftl_expand_params* p, const float* positions, float* out,
                           int64_t out_capacity, void* stream);
 
+/* -- Marschner lookup tables (src/fx/marschner.{h,cc}) ----------------------- */
+
+/* Marschner::ShadingParameters_t (marschner.h:31-46): fibre (eta,
+   absorption, eccentricity), surface (ar = longitudinal shift, br = width,
+   degrees) and glint settings. */
+typedef struct ftl_marschner_params {
+  float eta, absorption, eccentricity;
+  float ar, br;
+  float glint_scale, azimuthal_width, delta_caustic, delta_hm;
+} ftl_marschner_params;
+
+/* The reference defaults: 1.55, 0.20, 0.85, -5, 5, 0.5, 10, 0.2, 0.5. */
+void ftl_marschner_params_reference(ftl_marschner_params* p);
+
+/* Marschner::generate (marschner.cc:35-66): the two res x res tables into
+   out_m and out_n (device float4 [res*res] each, texel (x, y) at y*res + x),
+   asynchronous on stream:
+     M (cs_marschner_m.glsl:33-62) over sin theta_i = 2x/res - 1, sin theta_r =
+       2y/res - 1: the R, TT, TRT longitudinal gaussians of the half angle
+       (shifts (1, -0.5, -1.5) ar, widths (1, 0.5, 2) br) and cos theta_d;
+     N (cs_marschner_n.glsl:32-84, the generic branch) over cos phi_d = 2x/res
+       - 1, cos theta_d = 2y/res - 1: Np(0), Np(1), Np(2) of
+       inc_marschner_n.glsl:97-131 (cubic roots of inc_solver.glsl, Fresnel of
+       inc_fresnel.glsl), and 1.
+   FP32 texels (the reference stores RGBA16F).  Integer powers are products
+   (GLSL's pow is undefined for negative bases, which the reference reaches
+   where cos theta_d < 0); Pi is the reference's 3.141564. */
+ftl_status ftl_marschner_luts(ftl_ctx* ctx, const ftl_marschner_params* p, int res, float* out_m, float* out_n,
+                              void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,8 @@
 //   src/shaders/shared/inc_maths.glsl:107-115,210-228,239-241,263-266
 //                                          (render-side strand expansion, the
 //                                           stream pass of src/fx/hair.cc:144-170)
+//   src/shaders/hair/marschner/*.glsl, src/shaders/shared/inc_{solver,fresnel}.glsl
+//                                          (Marschner lookup tables, src/fx/marschner.cc:35-66)
 //   src/fx/hair.cc:236-328                 (init_simulation: positions, velocities,
 //                                           "curly" tangents; glm::simplex restated
 //                                           from the published webgl-noise algorithm)
@@ -740,5 +742,182 @@ void oracle_reference_tangents_f32(int nstrands, int npoints, const float* norma
     }
   }
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Marschner lookup tables (cs_marschner_m.glsl, cs_marschner_n.glsl with
+// inc_marschner_n.glsl, inc_solver.glsl, inc_fresnel.glsl), templated on the
+// precision: float (the FP32 restatement) and double (the accuracy check).
+// Integer powers are products; Pi() is the reference's 3.141564.
+
+namespace {
+
+template <typename T>
+struct MRoots {
+  T r[3];
+  int n;
+};
+
+template <typename T>
+MRoots<T> m_linear(T a, T b) {
+  MRoots<T> c{{0, 0, 0}, 0};
+  if (std::fabs(a) > T(1e-6f)) {
+    c.r[0] = -b / a;
+    c.n = 1;
+  }
+  return c;
+}
+
+template <typename T>
+MRoots<T> m_quadratic(T a, T b, T c) {
+  if (std::fabs(a) < T(1e-6f)) return m_linear(b, c);
+  T delta = b * b - T(4) * a * c;
+  MRoots<T> q{{0, 0, 0}, 0};
+  if (delta < T(0)) return q;
+  delta = std::sqrt(delta);
+  q.r[0] = (-b + delta) / (T(2) * a);
+  q.r[1] = (-b - delta) / (T(2) * a);
+  q.n = 1 + (delta >= T(1e-6f) ? 1 : 0);
+  return q;
+}
+
+template <typename T>
+MRoots<T> m_cubic(T A, T B, T C, T Dd) {
+  if (std::fabs(A) < T(1e-6f)) return m_quadratic(B, C, Dd);
+  const T a = B / A, b = C / A, c = Dd / A;
+  MRoots<T> r{{0, 0, 0}, 0};
+  if (std::fabs(c) < T(1e-6f)) {
+    r = m_quadratic(T(1), a, b);
+    r.r[r.n] = T(0);
+    r.n += 1;
+    return r;
+  }
+  const T Q = (T(3) * b - a * a) / T(9);
+  const T R = (T(9) * a * b - T(27) * c - T(2) * a * a * a) / T(54);
+  const T Q3 = Q * Q * Q;
+  const T D = Q3 + R * R;
+  const T third = a / T(3);
+  const T pi = T(3.141564f);
+  if (D > T(0)) {
+    const T sd = std::sqrt(D);
+    const T u = R + sd, v = R - sd;
+    const T s = (u > T(0) ? T(1) : (u < T(0) ? T(-1) : T(0))) * std::pow(std::fabs(u), T(0.333f));
+    const T t = (v > T(0) ? T(1) : (v < T(0) ? T(-1) : T(0))) * std::pow(std::fabs(v), T(0.333f));
+    r.r[0] = s + t - third;
+    r.n = 1;
+  } else {
+    const T theta = std::acos(R * (T(1) / std::sqrt(-Q3)));
+    const T tq = T(2) * std::sqrt(-Q);
+    r.r[0] = std::fma(tq, std::cos(theta / T(3)), -third);
+    r.r[1] = std::fma(tq, std::cos((theta + T(2) * pi) / T(3)), -third);
+    r.r[2] = std::fma(tq, std::cos((theta + T(4) * pi) / T(3)), -third);
+    r.n = 3;
+  }
+  return r;
+}
+
+template <typename T>
+T m_power_ratio(T ratio, T nA, T nB, T cosI, T sinI) {
+  const T e = ratio * sinI;
+  const T s2 = e * e;
+  if (s2 > T(1)) return T(1);
+  const T cosT = std::sqrt(T(1) - s2);
+  const T A = nA * cosI, B = nB * cosT;
+  const T R = (A - B) / (A + B);
+  return std::min(T(1), R * R);
+}
+
+template <typename T>
+T m_fresnel(T e0, T eperp, T epar, T c, T s) {
+  return m_power_ratio(e0 / eperp, e0, eperp, c, s) * (T(1) - T(0.5f)) + m_power_ratio(e0 / epar, epar, e0, c, s) * T(0.5f);
+}
+
+template <typename T>
+T m_absorption(int p, T absorption, T eperp, T epar, T cosG, T sinG) {
+  if (p == 0) return m_fresnel(T(1), eperp, epar, cosG, sinG);
+  const T sinT = sinG / eperp;
+  const T cosT = std::cos(std::asin(sinT));
+  const T fi = m_fresnel(T(1), eperp, epar, cosG, sinG);
+  const T ft = m_fresnel(T(1), T(1) / eperp, T(1) / epar, cosT, sinT);
+  const T t = std::exp(T(-4) * absorption * (cosT * cosT));
+  T a = (T(1) - fi) * (T(1) - fi);
+  for (int k = 0; k < p - 1; ++k) a *= ft;
+  for (int k = 0; k < p; ++k) a *= t;
+  return a;
+}
+
+template <typename T>
+T m_np(int p, T absorption, T eperp, T epar, T c, T phi) {
+  const T A = T(p) * c * T(-0.25801227547f);
+  const T C = T(p) * c * T(1.90985931710f) - T(2);
+  const T D = T(p) * T(3.141564f) - phi;
+  const MRoots<T> roots = m_cubic(A, T(0), C, D);
+  T L = T(0);
+  for (int i = 0; i < roots.n; ++i) {
+    const T g = roots.r[i];
+    const T a = m_absorption(p, absorption, eperp, epar, std::cos(g), std::sin(g));
+    const T d1 = (T(3) * A * (g * g) + C) / std::cos(g);
+    L += a * (T(1) / std::fabs(T(2) * d1));
+  }
+  return std::min(L, T(1));
+}
+
+template <typename T>
+T m_gauss(T sigma, T xmu) {
+  return std::exp(-(xmu * xmu) / (T(2) * sigma * sigma)) / (T(2.5066282f) * std::fabs(sigma));
+}
+
+template <typename T>
+void m_luts(int res, float eta_f, float absorption_f, float ar_f, float br_f, T* M, T* N) {
+  const T inv = T(1.0f / (float)res);
+  const T eta = eta_f, absorption = absorption_f, ar = ar_f, br = br_f;
+  for (int y = 0; y < res; ++y)
+    for (int x = 0; x < res; ++x) {
+      T* m = M + 4 * ((size_t)y * res + x);
+      const T sI = T(2) * T(x) * inv - T(1), sR = T(2) * T(y) * inv - T(1);
+      const T tI = std::asin(sI), tR = std::asin(sR);
+      const T tH = (tI + tR) / T(2), tD = (tI - tR) / T(2);
+      const T deg = tH * (T(180.0f) / T(3.14159265358979323846f));
+      m[0] = m_gauss(T(1) * br, deg - T(1) * ar);
+      m[1] = m_gauss(T(0.5f) * br, deg - T(-0.5f) * ar);
+      m[2] = m_gauss(T(2) * br, deg - T(-1.5f) * ar);
+      m[3] = std::cos(tD);
+      T* n = N + 4 * ((size_t)y * res + x);
+      const T cPhi = T(2) * T(x) * inv - T(1), cTh = T(2) * T(y) * inv - T(1);
+      const T s2 = T(1) - cTh * cTh;
+      const T eta2 = eta * eta;
+      const T eperp = std::sqrt(eta2 - s2) / cTh;
+      const T epar = eta2 / eperp;
+      const T phi = std::acos(cPhi);
+      const T c = std::asin(T(1) / eperp);
+      n[0] = m_np(0, absorption, eperp, epar, c, phi);
+      n[1] = m_np(1, absorption, eperp, epar, c, phi);
+      n[2] = m_np(2, absorption, eperp, epar, c, phi);
+      n[3] = T(1);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_marschner_f32(int res, float eta, float absorption, float ar, float br, float* M, float* N) {
+  m_luts<float>(res, eta, absorption, ar, br, M, N);
+}
+void oracle_marschner_f64(int res, float eta, float absorption, float ar, float br, double* M, double* N) {
+  m_luts<double>(res, eta, absorption, ar, br, M, N);
+}
+// Np(p) at one (eta_perp, eta_par, c, phi), and the roots of the cubic
+// A g^3 + C g + D (count returned; roots into r[3]), for the tests.
+float oracle_marschner_np_f32(int p, float absorption, float eperp, float epar, float c, float phi) {
+  return m_np<float>(p, absorption, eperp, epar, c, phi);
+}
+int oracle_cubic_f64(double A, double B, double C, double D, double* r) {
+  const MRoots<double> m = m_cubic<double>(A, B, C, D);
+  for (int i = 0; i < 3; ++i) r[i] = m.r[i];
+  return m.n;
+}
+float oracle_fresnel_f32(float eperp, float epar, float c, float s) { return m_fresnel<float>(1.0f, eperp, epar, c, s); }
 
 }  // extern "C"

@@ -69,6 +69,14 @@ def lib():
         L.oracle_simplex2_f32.restype = C.c_float
         L.oracle_simplex2_f32.argtypes = [C.c_float, C.c_float]
         L.oracle_reference_tangents_f32.argtypes = [C.c_int, C.c_int, vp, C.c_float, C.c_int, vp]
+        L.oracle_marschner_f32.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, vp, vp]
+        L.oracle_marschner_f64.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, vp, vp]
+        L.oracle_marschner_np_f32.restype = C.c_float
+        L.oracle_marschner_np_f32.argtypes = [C.c_int] + [C.c_float] * 5
+        L.oracle_cubic_f64.restype = C.c_int
+        L.oracle_cubic_f64.argtypes = [C.c_double] * 4 + [vp]
+        L.oracle_fresnel_f32.restype = C.c_float
+        L.oracle_fresnel_f32.argtypes = [C.c_float] * 4
         _lib = L
     return _lib
 
@@ -245,6 +253,29 @@ def reference_tangents_f32(normals, npoints, maxlength, curly=True):
     out = np.empty((n, npoints, 4), np.float32)
     lib().oracle_reference_tangents_f32(n, npoints, _p(normals), f32(maxlength), 1 if curly else 0, _p(out))
     return out
+
+
+def marschner_luts(res=128, eta=1.55, absorption=0.2, ar=-5.0, br=5.0, dtype=np.float32):
+    """Marschner M and N tables [res, res, 4] (float32 restatement, or float64)."""
+    M = np.empty((res, res, 4), dtype)
+    N = np.empty_like(M)
+    fn = lib().oracle_marschner_f32 if dtype == np.float32 else lib().oracle_marschner_f64
+    fn(res, f32(eta), f32(absorption), f32(ar), f32(br), _p(M), _p(N))
+    return M, N
+
+
+def marschner_np_f32(p, absorption, eperp, epar, c, phi):
+    return float(lib().oracle_marschner_np_f32(p, *(f32(v) for v in (absorption, eperp, epar, c, phi))))
+
+
+def cubic_f64(A, B, C, D):
+    r = np.zeros(3)
+    n = lib().oracle_cubic_f64(A, B, C, D, _p(r))
+    return r[:n]
+
+
+def fresnel_f32(eperp, epar, c, s):
+    return float(lib().oracle_fresnel_f32(f32(eperp), f32(epar), f32(c), f32(s)))
 
 
 def max_threads():
