@@ -48,6 +48,28 @@ ftl_status ftl_hip_fail(hipError_t e, const char* what) {
 // ---------------------------------------------------------------------------
 // memory
 
+// A caller's device pointer: it must point into device, managed or pinned
+// host memory, and its allocation must hold `bytes` from it, so a kernel
+// launched on it stays in bounds.
+ftl_status ftl_check_device_range(const void* p, size_t bytes, const char* what) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess || a.type == hipMemoryTypeUnregistered) {
+    (void)hipGetLastError();
+    return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "%s is not device memory", what);
+  }
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) == hipSuccess && base) {
+    const size_t avail = size - (size_t)((const char*)p - (const char*)base);
+    if (bytes > avail)
+      return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "%s: its allocation holds %zu bytes from the pointer, the call needs %zu",
+                      what, avail, bytes);
+  } else {
+    (void)hipGetLastError();
+  }
+  return FTL_OK;
+}
+
 ftl_status ftl_dev_alloc(ftl_ctx* ctx, void** p, size_t bytes, const char* what) {
   *p = nullptr;
   if (bytes == 0) return FTL_OK;
@@ -697,6 +719,10 @@ extern "C" ftl_status ftl_diag_copy(ftl_ctx* ctx, const void* src, void* dst, si
   if (!ctx || !src || !dst || bytes % 16) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "bad copy arguments");
   hipError_t e;
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
+  ftl_status st;
+  if ((st = ftl_check_device_range(src, bytes, "src")) != FTL_OK ||
+      (st = ftl_check_device_range(dst, bytes, "dst")) != FTL_OK)
+    return st;
   if ((e = ftl_launch_copy(src, dst, bytes / 16, ctx->ncu, (hipStream_t)stream)) != hipSuccess)
     return ftl_hip_fail(e, "copy launch");
   return FTL_OK;

@@ -77,6 +77,8 @@ ftl_status ftl_fail(ftl_status st, const char* fmt, ...);
 ftl_status ftl_groom_create_uninitialised(ftl_ctx* ctx, const ftl_groom_desc* d, ftl_groom** out);
 ftl_status ftl_hip_fail(hipError_t e, const char* what);
 ftl_status ftl_dev_alloc(ftl_ctx* ctx, void** p, size_t bytes, const char* what);
+// Validates a caller's device pointer before a kernel uses `bytes` from it.
+ftl_status ftl_check_device_range(const void* p, size_t bytes, const char* what);
 void ftl_dev_free(ftl_ctx* ctx, void* p, size_t bytes);
 // Fills the kernel descriptor of one step of g.
 ftl_status ftl_make_step(ftl_groom* g, float dt, FtlStep* A);

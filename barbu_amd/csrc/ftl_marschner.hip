@@ -202,6 +202,11 @@ extern "C" ftl_status ftl_marschner_luts(ftl_ctx* ctx, const ftl_marschner_param
   if (res < 2 || res > 8192) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "ftl_marschner_luts: resolution %d", res);
   hipError_t he;
   if ((he = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(he, "hipSetDevice");
+  const size_t bytes = (size_t)res * res * sizeof(float4);
+  ftl_status st;
+  if ((st = ftl_check_device_range(out_m, bytes, "out_m")) != FTL_OK ||
+      (st = ftl_check_device_range(out_n, bytes, "out_n")) != FTL_OK)
+    return st;
   LutArgs A{(float4*)out_m, (float4*)out_n, res, 1.0f / (float)res, p->eta, p->absorption, p->ar, p->br};
   const int n = res * res;
   hipLaunchKernelGGL(ftl_marschner_kernel, dim3((unsigned)((n + 255) / 256), 2), dim3(256), 0, (hipStream_t)stream, A);

@@ -288,6 +288,11 @@ extern "C" ftl_status ftl_expand_run(ftl_expand* e, const ftl_expand_params* p, 
                     (long long)nthreads);
   hipError_t he;
   if ((he = hipSetDevice(e->ctx->device)) != hipSuccess) return ftl_hip_fail(he, "hipSetDevice");
+  ftl_status st;
+  if ((st = ftl_check_device_range(positions, (size_t)e->nstrands * e->npoints * sizeof(float4), "positions")) !=
+          FTL_OK ||
+      (st = ftl_check_device_range(out, (size_t)need * sizeof(float4), "out")) != FTL_OK)
+    return st;
   ExpandArgs A;
   A.pos = (const float4*)positions;
   A.tan = e->d_tan;

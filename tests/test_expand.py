@@ -221,6 +221,13 @@ def test_gpu_errors(ctx):
         e.run(p, buf.ptr, n - 1)
     with pytest.raises(ba.FtlError):
         e.run(p, buf.ptr, n, ba.expand_params(nsubsegments=65))
+    host = np.zeros((pos.shape[0], pos.shape[1], 4), np.float32)
+    with pytest.raises(ba.FtlError):       # host memory, checked before the launch
+        e.run(host.ctypes.data, buf.ptr, n)
+    small = ba.DeviceBuffer(ctx, (n // 2) * 16)
+    with pytest.raises(ba.FtlError):       # the allocation is shorter than the capacity claimed
+        e.run(p, small.ptr, n)
+    small.close()
     buf.close()
     e.close()
     g.close()
