@@ -22,7 +22,7 @@ FTL_ROOT_REST = 1
 FTL_ROOT_SKINNED = 2
 FTL_MAX_COLLIDERS = 4
 FTL_MAX_POINTS = 64
-FTL_WIND_GRID = 16
+FTL_WIND_GRID = 8
 
 
 class Params(C.Structure):

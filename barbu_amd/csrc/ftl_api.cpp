@@ -413,7 +413,6 @@ extern "C" ftl_status ftl_groom_set_lanes(ftl_groom* g, int lanes) {
   return FTL_OK;
 }
 
-// Uploads the wind lattice when the wind parameters changed.
 // Uploads the wind lattice when the wind parameters changed, in the device
 // layout of the kernels: per node (v.x, v.y, v.z, d.x), (d.y, d.z, 0, 0) with
 // d = v(x+1) - v(x) along the periodic x axis.

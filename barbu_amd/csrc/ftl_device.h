@@ -160,7 +160,8 @@ struct WindTaps {
   float s0, s1, s2;
 };
 
-__device__ __forceinline__ WindTaps wind_fetch(const WindArgs& W, float3 x) {
+// T: the lattice in global memory (W.T) or a copy of it in LDS.
+__device__ __forceinline__ WindTaps wind_fetch_from(const float4* __restrict__ T, const WindArgs& W, float3 x) {
   // The lattice side is the constant FTL_WIND_GRID (a power of two): corner
   // indices are masks and shifts, and every offset fits 32 bits, so the
   // gathers address the table from its scalar base.
@@ -180,7 +181,6 @@ __device__ __forceinline__ WindTaps wind_fetch(const WindArgs& W, float3 x) {
   t.s2 = (f2 * f2) * __builtin_fmaf(-2.0f, f2, 3.0f);
   const unsigned n[4] = {(z0 * G + y0) * G + x0, (z0 * G + y1) * G + x0, (z1 * G + y0) * G + x0,
                          (z1 * G + y1) * G + x0};
-  const float4* __restrict__ T = (const float4*)W.T;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     t.a[e] = T[2u * n[e]];
@@ -188,6 +188,10 @@ __device__ __forceinline__ WindTaps wind_fetch(const WindArgs& W, float3 x) {
     t.b[e] = make_float2(h.x, h.y);
   }
   return t;
+}
+
+__device__ __forceinline__ WindTaps wind_fetch(const WindArgs& W, float3 x) {
+  return wind_fetch_from((const float4*)W.T, W, x);
 }
 
 __device__ __forceinline__ float3 wind_blend(const WindTaps& t) {

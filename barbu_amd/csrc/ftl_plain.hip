@@ -7,6 +7,9 @@
 #ifndef FTL_CLASS_ID
 #error "build with -DFTL_CLASS_ID=<launch class>"
 #endif
+#ifndef FTL_WIND_LDS
+#define FTL_WIND_LDS 1
+#endif
 
 namespace {
 
@@ -48,6 +51,42 @@ void ftl_plain_kernel(FtlStep arg) {
   ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead>(A, blockIdx.x * (FTL_WAVE / CL::G), sh);
 }
 
+// With wind: eight tiles (waves) per workgroup share one LDS copy of the wind
+// lattice, so the load phase's gathers are LDS reads (their waits are
+// independent of the HBM row loads in flight) instead of global gathers.  The
+// workgroups are persistent: wave w of workgroup b steps tiles 8b + w,
+// 8b + w + 8 grid, ...  Used by the classes that run at most four waves per
+// SIMD, where the workgroups that make that occupancy (one per two waves per
+// SIMD) fit the LDS with eight stages and the lattice each.
+constexpr int kWindWaves = 8;
+constexpr int kLatticeF4 = 2 * FTL_WIND_GRID * FTL_WIND_GRID * FTL_WIND_GRID;
+constexpr size_t kWindGroupLds =
+    kWindWaves * sizeof(ftl_tile_impl::TileLds<CL::M, CL::G, CL::GU>) + kLatticeF4 * sizeof(float4);
+constexpr bool kWindLds = (kWavesPerEu == 4 && 2 * kWindGroupLds <= 160 * 1024) ||
+                          (kWavesPerEu == 2 && kWindGroupLds <= 160 * 1024);
+
+__global__ __launch_bounds__(FTL_WAVE * kWindWaves) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
+void ftl_plain_wind_kernel(FtlStep arg) {
+  (void)arg;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const FtlStep& A = *(const FtlStep*)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+  const FtlStep& A = arg;
+#endif
+  // (a one-element placeholder for the classes that do not use this form)
+  __shared__ float4 lattice[kWindLds ? kLatticeF4 : 1];
+  __shared__ ftl_tile_impl::TileLds<CL::M, CL::G, CL::GU> sh[kWindLds ? kWindWaves : 1];
+  if constexpr (kWindLds) {
+    for (int i = (int)threadIdx.x; i < kLatticeF4; i += FTL_WAVE * kWindWaves) lattice[i] = A.wind[i];
+    __syncthreads();
+    constexpr int spt = FTL_WAVE / CL::G;
+    const int ntiles = (A.nstrands + spt - 1) / spt;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / FTL_WAVE);
+    for (int t = (int)blockIdx.x * kWindWaves + w; t < ntiles; t += (int)gridDim.x * kWindWaves)
+      ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead, true>(A, t * spt, sh[w], lattice);
+  }
+}
+
 }  // namespace
 
 #define FTL_PASTE2(a, b) a##b
@@ -57,6 +96,25 @@ hipError_t FTL_PASTE(ftl_launch_plain_, FTL_CLASS_ID)(const FtlStep& A, hipStrea
   constexpr int spt = FTL_WAVE / CL::G;
   const int nblocks = (A.nstrands + spt - 1) / spt;
   if (nblocks <= 0) return hipSuccess;
+  if (kWindLds && FTL_WIND_LDS && A.wind_grid) {
+    // resident workgroups of the device (occupancy x CUs), queried once
+    static int resident = 0;
+    if (resident == 0) {
+      int dev = 0, ncu = 0, per_cu = 0;
+      hipError_t e;
+      if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+      if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+      if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ftl_plain_wind_kernel, FTL_WAVE * kWindWaves,
+                                                             0)) != hipSuccess)
+        return e;
+      if (per_cu <= 0) return hipErrorInvalidConfiguration;
+      resident = ncu * per_cu;
+    }
+    const int need = (nblocks + kWindWaves - 1) / kWindWaves;
+    hipLaunchKernelGGL(ftl_plain_wind_kernel, dim3(need < resident ? need : resident), dim3(FTL_WAVE * kWindWaves),
+                       0, stream, A);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(ftl_plain_kernel, dim3(nblocks), dim3(FTL_WAVE), 0, stream, A);
   return hipGetLastError();
 }

@@ -147,6 +147,17 @@ __device__ __forceinline__ float3 root_source(const FtlStep& A, int s) {
             __builtin_fmaf(w2, c.z, __builtin_fmaf(w1, bb.z, w0 * a.z)));
 }
 
+// Orders the tile's LDS stage accesses across the lanes of its wave.  A tile
+// is one wave and only that wave touches its stage, so no workgroup barrier
+// is needed (several tiles may share a workgroup): LDS instructions of one
+// wave execute in issue order, and the fences keep the compiler from moving
+// stage accesses across this point.
+__device__ __forceinline__ void tile_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Writes chunk c of the tile (C points per lane, staged in LDS) to dst in
 // the coalesced order of the load.  Called by the whole wave.  gsel >= 0
 // writes only the points owned by lanes with g == gsel (the lanes of a
@@ -159,7 +170,7 @@ __device__ __forceinline__ void flush_chunk(float4* __restrict__ dst, int strand
   // until the stores would cost registers for the whole step.
   int ln = lane;
   asm volatile("" : "+v"(ln));
-  __syncthreads();
+  tile_sync();
 #pragma unroll
   for (int k = 0; k < C; ++k) {
     const int q = k * FTL_WAVE + ln;
@@ -169,7 +180,7 @@ __device__ __forceinline__ void flush_chunk(float4* __restrict__ dst, int strand
     const int cp = (r % G) * M + c * C + j;
     if (s < nstr && (!GUARD || cp < N) && (gsel < 0 || r % G == gsel)) dst[(size_t)s * N + cp] = stage[slot<C>(r, j)];
   }
-  __syncthreads();
+  tile_sync();
 }
 
 // One sweep of DistanceConstraint over this lane's points (G == 1), fused with
@@ -352,8 +363,11 @@ struct TileLds : TileLens<M, G == 1> {
 
 // PD: rows of the load phase whose loads are in flight ahead of the row being
 // integrated (each costs 7 VGPRs; the forms with a high occupancy target use 1).
-template <int M, int G, bool GUARD, int PD_ = FTL_LOAD_AHEAD>
-__device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<M, G, GUARD>& sh) {
+// LW: the wind gathers read wl, the groom's wind lattice copied to LDS by the
+// workgroup, instead of the lattice in global memory.
+template <int M, int G, bool GUARD, int PD_ = FTL_LOAD_AHEAD, bool LW = false>
+__device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<M, G, GUARD>& sh,
+                                         const float4* wl = nullptr) {
   float4* __restrict__ stage = sh.stage;
   constexpr int C = Tile<M, G, GUARD>::C;
   // Opaque: in the persistent kernels nothing lane-derived is hoisted out of
@@ -425,7 +439,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     const int cp = (r % G) * M + c * C + j;
     const float4 x = xr[i % (PD + 1)], v = vr[i % (PD + 1)];
     WindTaps wt;
-    if (wind) wt = wind_fetch(W, f3(x.x, x.y, x.z));
+    if (wind) wt = LW ? wind_fetch_from(wl, W, f3(x.x, x.y, x.z)) : wind_fetch(W, f3(x.x, x.y, x.z));
     __builtin_amdgcn_sched_barrier(0);
     if (i + PD < M) load_row(i + PD);
     __builtin_amdgcn_sched_barrier(0);
@@ -441,14 +455,14 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     stage[slot<C>(r, j)] = make_float4(pr.x, pr.y, pr.z, x.w);
     __builtin_amdgcn_sched_barrier(0);
     if (i % C == C - 1) {
-      __syncthreads();
+      tile_sync();
 #pragma unroll
       for (int k = 0; k < C; ++k) {
         const float4 t = stage[slot<C>(lane, k)];
         p[c * C + k] = f3(t.x, t.y, t.z);
         l.set(c * C + k, t.w);
       }
-      __syncthreads();
+      tile_sync();
       if (c == 0 && owns_root) {
         // root: p0 = M * source (:190-193); the pipelined forms keep u0 = p0 - x0
         const float3 x0 = p[0];

@@ -37,7 +37,7 @@ extern "C" {
 #define FTL_ABI_VERSION 1
 #define FTL_MAX_COLLIDERS 4
 #define FTL_MAX_POINTS 64
-#define FTL_WIND_GRID 16   /* side of the periodic wind lattice */
+#define FTL_WIND_GRID 8    /* side of the periodic wind lattice */
 
 typedef int ftl_status;
 enum {
