@@ -1,5 +1,9 @@
-// Plain (one groom) FTL step kernel for one launch class, selected at build
-// time with -DFTL_CLASS_ID=<id>: one wave64 workgroup per tile of strands.
+// Plain (one groom) FTL step kernels for one launch class, selected at build
+// time with -DFTL_CLASS_ID=<id>: one wave64 workgroup per tile of strands, or,
+// for launches with wind, persistent workgroups of eight tiles sharing an LDS
+// copy of the wind lattice; and the tile-list kernel of batched frames.
+#include <atomic>
+
 #include "ftl_classes.h"
 #include "ftl_launch.h"
 #include "ftl_tile.h"
@@ -97,18 +101,24 @@ hipError_t FTL_PASTE(ftl_launch_plain_, FTL_CLASS_ID)(const FtlStep& A, hipStrea
   const int nblocks = (A.nstrands + spt - 1) / spt;
   if (nblocks <= 0) return hipSuccess;
   if (kWindLds && FTL_WIND_LDS && A.wind_grid) {
-    // resident workgroups of the device (occupancy x CUs), queried once
-    static int resident = 0;
+    // Resident workgroups of the current device (occupancy x CUs), queried
+    // once per device.  A grid of any size covers every tile; this one
+    // only sizes it.
+    constexpr int kMaxDev = 64;
+    static std::atomic<int> resident_of[kMaxDev];
+    int dev = 0;
+    hipError_t e;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    int resident = (dev >= 0 && dev < kMaxDev) ? resident_of[dev].load(std::memory_order_relaxed) : 0;
     if (resident == 0) {
-      int dev = 0, ncu = 0, per_cu = 0;
-      hipError_t e;
-      if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+      int ncu = 0, per_cu = 0;
       if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
       if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ftl_plain_wind_kernel, FTL_WAVE * kWindWaves,
                                                              0)) != hipSuccess)
         return e;
-      if (per_cu <= 0) return hipErrorInvalidConfiguration;
+      if (per_cu <= 0 || ncu <= 0) return hipErrorInvalidConfiguration;
       resident = ncu * per_cu;
+      if (dev >= 0 && dev < kMaxDev) resident_of[dev].store(resident, std::memory_order_relaxed);
     }
     const int need = (nblocks + kWindWaves - 1) / kWindWaves;
     hipLaunchKernelGGL(ftl_plain_wind_kernel, dim3(need < resident ? need : resident), dim3(FTL_WAVE * kWindWaves),
