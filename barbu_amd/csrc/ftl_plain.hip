@@ -1,9 +1,7 @@
 // Plain (one groom) FTL step kernels for one launch class, selected at build
 // time with -DFTL_CLASS_ID=<id>: one wave64 workgroup per tile of strands, or,
-// for launches with wind, persistent workgroups of eight tiles sharing an LDS
-// copy of the wind lattice; and the tile-list kernel of batched frames.
-#include <atomic>
-
+// for launches with wind, workgroups of eight tiles sharing an LDS copy of the
+// wind lattice; and the tile-list kernel of batched frames.
 #include "ftl_classes.h"
 #include "ftl_launch.h"
 #include "ftl_tile.h"
@@ -57,11 +55,10 @@ void ftl_plain_kernel(FtlStep arg) {
 
 // With wind: eight tiles (waves) per workgroup share one LDS copy of the wind
 // lattice, so the load phase's gathers are LDS reads (their waits are
-// independent of the HBM row loads in flight) instead of global gathers.  The
-// workgroups are persistent: wave w of workgroup b steps tiles 8b + w,
-// 8b + w + 8 grid, ...  Used by the classes that run at most four waves per
-// SIMD, where the workgroups that make that occupancy (one per two waves per
-// SIMD) fit the LDS with eight stages and the lattice each.
+// independent of the HBM row loads in flight) instead of global gathers.
+// Wave w of workgroup b steps tile 8b + w.  Used by the classes that run at
+// most four waves per SIMD, where the workgroups that make that occupancy (one
+// per two waves per SIMD) fit the LDS with eight stages and the lattice each.
 constexpr int kWindWaves = 8;
 constexpr int kLatticeF4 = 2 * FTL_WIND_GRID * FTL_WIND_GRID * FTL_WIND_GRID;
 constexpr size_t kWindGroupLds =
@@ -86,8 +83,8 @@ void ftl_plain_wind_kernel(FtlStep arg) {
     constexpr int spt = FTL_WAVE / CL::G;
     const int ntiles = (A.nstrands + spt - 1) / spt;
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / FTL_WAVE);
-    for (int t = (int)blockIdx.x * kWindWaves + w; t < ntiles; t += (int)gridDim.x * kWindWaves)
-      ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead, true>(A, t * spt, sh[w], lattice);
+    const int t = (int)blockIdx.x * kWindWaves + w;
+    if (t < ntiles) ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead, true>(A, t * spt, sh[w], lattice);
   }
 }
 
@@ -101,27 +98,7 @@ hipError_t FTL_PASTE(ftl_launch_plain_, FTL_CLASS_ID)(const FtlStep& A, hipStrea
   const int nblocks = (A.nstrands + spt - 1) / spt;
   if (nblocks <= 0) return hipSuccess;
   if (kWindLds && FTL_WIND_LDS && A.wind_grid) {
-    // Resident workgroups of the current device (occupancy x CUs), queried
-    // once per device.  A grid of any size covers every tile; this one
-    // only sizes it.
-    constexpr int kMaxDev = 64;
-    static std::atomic<int> resident_of[kMaxDev];
-    int dev = 0;
-    hipError_t e;
-    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-    int resident = (dev >= 0 && dev < kMaxDev) ? resident_of[dev].load(std::memory_order_relaxed) : 0;
-    if (resident == 0) {
-      int ncu = 0, per_cu = 0;
-      if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-      if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ftl_plain_wind_kernel, FTL_WAVE * kWindWaves,
-                                                             0)) != hipSuccess)
-        return e;
-      if (per_cu <= 0 || ncu <= 0) return hipErrorInvalidConfiguration;
-      resident = ncu * per_cu;
-      if (dev >= 0 && dev < kMaxDev) resident_of[dev].store(resident, std::memory_order_relaxed);
-    }
-    const int need = (nblocks + kWindWaves - 1) / kWindWaves;
-    hipLaunchKernelGGL(ftl_plain_wind_kernel, dim3(need < resident ? need : resident), dim3(FTL_WAVE * kWindWaves),
+    hipLaunchKernelGGL(ftl_plain_wind_kernel, dim3((nblocks + kWindWaves - 1) / kWindWaves), dim3(FTL_WAVE * kWindWaves),
                        0, stream, A);
     return hipGetLastError();
   }

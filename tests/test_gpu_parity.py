@@ -150,18 +150,18 @@ def test_wind_and_rotating_emitter(ctx):
                 times=lambda k: (k + 1) * float(DT), label="wind")
 
 
-@pytest.mark.parametrize("nstrands,npoints,lanes", [(150000, 32, 2), (140000, 20, 1), (150000, 8, 2)])
-def test_wind_persistent_workgroups(ctx, nstrands, npoints, lanes):
-    """Plain launches with wind run persistent workgroups of eight tiles that
-    share an LDS copy of the lattice.  These grooms have more tiles than a
-    256-CU device holds at once (4,096 for the 2-lane classes, 2,048 for the
-    guarded 32-point class), so waves step several tiles each."""
+@pytest.mark.parametrize("nstrands,npoints,lanes", [(150100, 32, 2), (140000, 20, 1), (150100, 16, 2)])
+def test_wind_lds_workgroups(ctx, nstrands, npoints, lanes):
+    """Plain launches with wind run workgroups of eight tiles that share an
+    LDS copy of the lattice (the 2-lane 32- and 16-point classes and the
+    guarded 32-point class).  Hundreds of workgroups, the last one partial
+    and its last tile partial."""
     pos, vel, _ = ba.sphere_groom(nstrands, npoints, radius=1.0, length=0.3, seed=5)
     params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.45, wind_amplitude=2.0,
                             wind_frequency=2.0, wind_scroll=(0.5, 0.0, 0.25), wind_seed=7)
     spheres = [(0, 0, 0, 1), (0.3, 1.1, 0.0, 0.35)]
     run_compare(ctx, pos, vel, params, spheres, 2, lanes=lanes, times=lambda k: (k + 1) * float(DT),
-                label="wind persistent n=%d N=%d G=%d" % (nstrands, npoints, lanes))
+                label="wind lds n=%d N=%d G=%d" % (nstrands, npoints, lanes))
 
 
 @pytest.mark.parametrize("ncol", [0, 3, 4])
