@@ -35,7 +35,12 @@ COMMON = [
 def _units():
     units = []
     for k in range(NCLASSES):
-        units.append(("ftl_plain.hip", ["-DFTL_CLASS_ID=%d" % k], "plain_%d.o" % k))
+        # The tile's loops are fully unrolled (#pragma unroll) so the strand
+        # stays in registers; the 64-point class's bodies exceed LLVM's default
+        # size limit for that, and would otherwise keep the points in scratch.
+        # (The other classes compile to the same code with or without it.)
+        units.append(("ftl_plain.hip", ["-DFTL_CLASS_ID=%d" % k, "-mllvm", "-pragma-unroll-threshold=200000"],
+                      "plain_%d.o" % k))
     for src in ("ftl_misc.hip", "ftl_render.hip", "ftl_init.hip", "ftl_marschner.hip", "ftl_api.cpp", "ftl_batch_api.cpp", "ftl_autotune.cpp",
                 "ftl_scenario.cpp", "ftl_server.cpp"):
         units.append((src, [], os.path.splitext(src)[0] + ".o"))

@@ -226,12 +226,13 @@ int ftl_class_for(ftl_ctx* ctx, int npoints, int lanes, int mmax) {
       const FtlClassInfo& k = kFtlClasses[c];
       if (!k.guarded && k.g == g && k.m * k.g == npoints && k.m <= mmax) return c;
     }
-  // guarded, one lane per strand
+  // guarded, one lane per strand: the smallest that holds the strand
+  int best = -1;
   for (int c = 0; c < FTL_NCLASSES; ++c) {
     const FtlClassInfo& k = kFtlClasses[c];
-    if (k.guarded && k.m >= npoints && k.m <= mmax) return c;
+    if (k.guarded && k.m >= npoints && k.m <= mmax && (best < 0 || k.m < kFtlClasses[best].m)) best = c;
   }
-  return -1;
+  return best;
 }
 
 extern "C" int ftl_lanes_for(ftl_ctx* ctx, int npoints) {

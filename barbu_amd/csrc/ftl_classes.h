@@ -25,10 +25,11 @@
   X(11, 8, 1, true, 6, 4)     \
   X(12, 16, 1, true, 4, 4)    \
   X(13, 32, 1, true, 2, 4)    \
-  X(14, 64, 1, true, 2, 4)    \
-  X(15, 4, 2, false, 7, 4)
+  X(14, 64, 1, true, 1, 8)    \
+  X(15, 4, 2, false, 7, 4)     \
+  X(16, 48, 1, true, 1, 8)
 
-#define FTL_NCLASSES 16
+#define FTL_NCLASSES 17
 
 struct FtlClassInfo {
   int m, g, guarded, waves, chunk;
