@@ -237,14 +237,19 @@ extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, i
   }
   for (int k = 0; k < 2; ++k) {
     if ((st = ftl_dev_alloc(ctx, (void**)&b->d_steps[k], sb, "batch descriptors")) != FTL_OK) return bail(st);
-    if ((e = hipHostMalloc((void**)&b->h_steps[k], sb, hipHostMallocDefault)) != hipSuccess)
+#ifndef FTL_KERNEL_UPLOAD
+#define FTL_KERNEL_UPLOAD 1
+#endif
+    if ((e = hipHostMalloc((void**)&b->h_steps[k], sb, FTL_KERNEL_UPLOAD ? hipHostMallocCoherent : hipHostMallocDefault)) !=
+        hipSuccess)
       return bail(ftl_hip_fail(e, "hipHostMalloc"));
     std::memset(b->h_steps[k], 0, sb);
     if ((e = hipEventCreateWithFlags(&b->done[k], hipEventDisableTiming)) != hipSuccess)
       return bail(ftl_hip_fail(e, "hipEventCreate"));
     if ((e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return bail(ftl_hip_fail(e, "hipStreamBeginCapture"));
-    hipError_t e1 = hipMemcpyAsync(b->d_steps[k], b->h_steps[k], sb, hipMemcpyHostToDevice, b->cap);
+    hipError_t e1 = FTL_KERNEL_UPLOAD ? ftl_launch_upload(b->h_steps[k], b->d_steps[k], sb, b->cap)
+                                      : hipMemcpyAsync(b->d_steps[k], b->h_steps[k], sb, hipMemcpyHostToDevice, b->cap);
     hipError_t e2 = e1 == hipSuccess
                         ? ftl_enqueue_runs(b->runs, b->d_steps[k], b->d_tiles, b->cap, &b->side, &b->side_ev, b->fork)
                         : hipSuccess;

@@ -20,4 +20,7 @@ hipError_t ftl_launch_tiles(int cls, const FtlStep* grooms, const FtlTile* tiles
 int ftl_tiles_blocks_per_cu(int cls);
 hipError_t ftl_launch_skin(int nverts, const float4* dq, const int4* joints, const float4* weights,
                            const float4* rest, float4* out, hipStream_t stream);
+// Copies bytes (a multiple of 16) from pinned, coherent host memory to the
+// device with a kernel on `stream`.
+hipError_t ftl_launch_upload(const void* host_src, void* dst, size_t bytes, hipStream_t stream);
 hipError_t ftl_launch_copy(const void* src, void* dst, size_t n16, int ncu, hipStream_t stream);

@@ -73,6 +73,15 @@ __global__ __launch_bounds__(256) void ftl_copy_kernel(const float4* __restrict_
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// Upload of a small host array in pinned, coherent host memory by a kernel on
+// the compute queue (a graph or a serving batch then needs no copy-engine
+// node ahead of its kernels).
+__global__ __launch_bounds__(256) void ftl_upload_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                         size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Host dispatch.
 
@@ -127,6 +136,17 @@ hipError_t ftl_launch_skin(int nverts, const float4* dq, const int4* joints, con
   if (nverts <= 0) return hipSuccess;
   hipLaunchKernelGGL(ftl_skin_kernel, dim3((nverts + 255) / 256), dim3(256), 0, stream, nverts, dq, joints,
                      weights, rest, out);
+  return hipGetLastError();
+}
+
+static_assert(sizeof(FtlStep) % 16 == 0 && sizeof(FtlTile) % 16 == 0, "descriptors upload in 16-byte units");
+
+hipError_t ftl_launch_upload(const void* host_src, void* dst, size_t bytes, hipStream_t stream) {
+  if (bytes % 16 != 0) return hipErrorInvalidValue;
+  const size_t n = bytes / 16;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(ftl_upload_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                     (const uint4*)host_src, (uint4*)dst, n);
   return hipGetLastError();
 }
 

@@ -153,6 +153,14 @@ void admit_waiting(ftl_server* s) {
   s->cv_work.notify_all();
 }
 
+// A batch's descriptors and tiles are uploaded by a kernel on the slot's
+// stream, reading pinned coherent host memory (no copy-engine transfer ahead
+// of the step kernels).
+#ifndef FTL_KERNEL_UPLOAD
+#define FTL_KERNEL_UPLOAD 1
+#endif
+constexpr unsigned kHostUpload = FTL_KERNEL_UPLOAD ? hipHostMallocCoherent : hipHostMallocDefault;
+
 hipError_t ensure_slot(ftl_server* s, Slot& sl, size_t nsteps, size_t ntiles, size_t nruns) {
   hipError_t e;
   if (nsteps > sl.steps_cap) {
@@ -162,7 +170,7 @@ hipError_t ensure_slot(ftl_server* s, Slot& sl, size_t nsteps, size_t ntiles, si
     sl.h_steps = nullptr;
     sl.d_steps = nullptr;
     sl.steps_cap = 0;
-    if ((e = hipHostMalloc((void**)&sl.h_steps, cap * sizeof(FtlStep), hipHostMallocDefault)) != hipSuccess) return e;
+    if ((e = hipHostMalloc((void**)&sl.h_steps, cap * sizeof(FtlStep), kHostUpload)) != hipSuccess) return e;
     if (ftl_dev_alloc(s->ctx, (void**)&sl.d_steps, cap * sizeof(FtlStep), "server descriptors") != FTL_OK)
       return hipErrorOutOfMemory;
     sl.steps_cap = cap;
@@ -174,7 +182,7 @@ hipError_t ensure_slot(ftl_server* s, Slot& sl, size_t nsteps, size_t ntiles, si
     sl.h_tiles = nullptr;
     sl.d_tiles = nullptr;
     sl.tiles_cap = 0;
-    if ((e = hipHostMalloc((void**)&sl.h_tiles, cap * sizeof(FtlTile), hipHostMallocDefault)) != hipSuccess) return e;
+    if ((e = hipHostMalloc((void**)&sl.h_tiles, cap * sizeof(FtlTile), kHostUpload)) != hipSuccess) return e;
     if (ftl_dev_alloc(s->ctx, (void**)&sl.d_tiles, cap * sizeof(FtlTile), "server tiles") != FTL_OK)
       return hipErrorOutOfMemory;
     sl.tiles_cap = cap;
@@ -235,11 +243,18 @@ ftl_status launch_batch(ftl_server* s, Slot& sl) {
   for (size_t i = 0; i < n; ++i)
     if ((st = ftl_make_step(grooms[i], dts[i], &sl.h_steps[i])) != FTL_OK) return st;
   std::memcpy(sl.h_tiles, tiles.data(), tiles.size() * sizeof(FtlTile));
-  if ((e = hipMemcpyAsync(sl.d_steps, sl.h_steps, n * sizeof(FtlStep), hipMemcpyHostToDevice, sl.s)) != hipSuccess)
-    return ftl_hip_fail(e, "server descriptor upload");
-  if ((e = hipMemcpyAsync(sl.d_tiles, sl.h_tiles, tiles.size() * sizeof(FtlTile), hipMemcpyHostToDevice, sl.s)) !=
-      hipSuccess)
-    return ftl_hip_fail(e, "server tile upload");
+  if (FTL_KERNEL_UPLOAD) {
+    if ((e = ftl_launch_upload(sl.h_steps, sl.d_steps, n * sizeof(FtlStep), sl.s)) != hipSuccess)
+      return ftl_hip_fail(e, "server descriptor upload");
+    if ((e = ftl_launch_upload(sl.h_tiles, sl.d_tiles, tiles.size() * sizeof(FtlTile), sl.s)) != hipSuccess)
+      return ftl_hip_fail(e, "server tile upload");
+  } else {
+    if ((e = hipMemcpyAsync(sl.d_steps, sl.h_steps, n * sizeof(FtlStep), hipMemcpyHostToDevice, sl.s)) != hipSuccess)
+      return ftl_hip_fail(e, "server descriptor upload");
+    if ((e = hipMemcpyAsync(sl.d_tiles, sl.h_tiles, tiles.size() * sizeof(FtlTile), hipMemcpyHostToDevice, sl.s)) !=
+        hipSuccess)
+      return ftl_hip_fail(e, "server tile upload");
+  }
   if ((e = ftl_enqueue_runs(runs, sl.d_steps, sl.d_tiles, sl.s, &sl.side, &sl.side_ev, sl.fork)) != hipSuccess)
     return ftl_hip_fail(e, "server batch launch");
   if ((e = hipEventRecord(sl.done, sl.s)) != hipSuccess) return ftl_hip_fail(e, "hipEventRecord");
