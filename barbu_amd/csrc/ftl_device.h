@@ -194,6 +194,36 @@ __device__ __forceinline__ WindTaps wind_fetch(const WindArgs& W, float3 x) {
   return wind_fetch_from((const float4*)W.T, W, x);
 }
 
+// The same taps from a half lattice L that keeps each node's first float4,
+// (v.x, v.y, v.z, d.x): d.y and d.z are formed here as v(x+1) - v(x), the
+// single IEEE subtraction that built them in the full table.
+__device__ __forceinline__ WindTaps wind_fetch_half(const float4* __restrict__ L, const WindArgs& W, float3 x) {
+  constexpr unsigned G = FTL_WIND_GRID, msk = FTL_WIND_GRID - 1;
+  const float u0 = __builtin_fmaf(x.x, W.freq, W.off[0]);
+  const float u1 = __builtin_fmaf(x.y, W.freq, W.off[1]);
+  const float u2 = __builtin_fmaf(x.z, W.freq, W.off[2]);
+  const float l0 = floorf(u0), l1 = floorf(u1), l2 = floorf(u2);
+  const float f0 = u0 - l0, f1 = u1 - l1, f2 = u2 - l2;
+  const unsigned i0 = (unsigned)(int)l0, i1 = (unsigned)(int)l1, i2 = (unsigned)(int)l2;
+  const unsigned x0 = i0 & msk, x1 = (i0 + 1u) & msk;
+  const unsigned y0 = i1 & msk, y1 = (i1 + 1u) & msk;
+  const unsigned z0 = i2 & msk, z1 = (i2 + 1u) & msk;
+  WindTaps t;
+  t.s0 = (f0 * f0) * __builtin_fmaf(-2.0f, f0, 3.0f);
+  t.s1 = (f1 * f1) * __builtin_fmaf(-2.0f, f1, 3.0f);
+  t.s2 = (f2 * f2) * __builtin_fmaf(-2.0f, f2, 3.0f);
+  const unsigned r[4] = {(z0 * G + y0) * G, (z0 * G + y1) * G, (z1 * G + y0) * G, (z1 * G + y1) * G};
+  float4 nb[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    t.a[e] = L[r[e] + x0];
+    nb[e] = L[r[e] + x1];
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) t.b[e] = make_float2(nb[e].y - t.a[e].y, nb[e].z - t.a[e].z);
+  return t;
+}
+
 __device__ __forceinline__ float3 wind_blend(const WindTaps& t) {
   const float3 c00 = wind_edge(t.a[0], t.b[0], t.s0);
   const float3 c10 = wind_edge(t.a[1], t.b[1], t.s0);

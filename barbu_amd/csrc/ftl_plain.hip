@@ -1,6 +1,6 @@
 // Plain (one groom) FTL step kernels for one launch class, selected at build
 // time with -DFTL_CLASS_ID=<id>: one wave64 workgroup per tile of strands, or,
-// for launches with wind, workgroups of eight tiles sharing an LDS copy of the
+// for launches with wind, workgroups of four tiles sharing an LDS copy of the
 // wind lattice; and the tile-list kernel of batched frames.
 #include "ftl_classes.h"
 #include "ftl_launch.h"
@@ -53,18 +53,28 @@ void ftl_plain_kernel(FtlStep arg) {
   ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead>(A, blockIdx.x * (FTL_WAVE / CL::G), sh);
 }
 
-// With wind: eight tiles (waves) per workgroup share one LDS copy of the wind
+// With wind: four tiles (waves) per workgroup share one LDS copy of the wind
 // lattice, so the load phase's gathers are LDS reads (their waits are
 // independent of the HBM row loads in flight) instead of global gathers.
-// Wave w of workgroup b steps tile 8b + w.  Used by the classes that run at
-// most four waves per SIMD, where the workgroups that make that occupancy (one
-// per two waves per SIMD) fit the LDS with eight stages and the lattice each.
-constexpr int kWindWaves = 8;
-constexpr int kLatticeF4 = 2 * FTL_WIND_GRID * FTL_WIND_GRID * FTL_WIND_GRID;
+// Wave w of workgroup b steps tile 4b + w.  The copy is the half lattice (each
+// node's first float4, 8 KB): four stages plus it make 40 KB, so the four
+// workgroups of a CU at four waves per SIMD fit its 160 KB.  (Eight-tile
+// workgroups with the whole lattice also fit, but a workgroup holds its LDS
+// until its slowest tile ends.)  Used by the classes that run at most four
+// waves per SIMD.
+#ifndef FTL_WIND_WAVES
+#define FTL_WIND_WAVES 4
+#endif
+constexpr int kWindWaves = FTL_WIND_WAVES;
+// eight-wave groups copy the whole lattice, four-wave groups its half lattice
+constexpr int kWindHalf = kWindWaves == 4;
+constexpr int kLatticeF4 = (kWindHalf ? 1 : 2) * FTL_WIND_GRID * FTL_WIND_GRID * FTL_WIND_GRID;
 constexpr size_t kWindGroupLds =
     kWindWaves * sizeof(ftl_tile_impl::TileLds<CL::M, CL::G, CL::GU>) + kLatticeF4 * sizeof(float4);
-constexpr bool kWindLds = (kWavesPerEu == 4 && 2 * kWindGroupLds <= 160 * 1024) ||
-                          (kWavesPerEu == 2 && kWindGroupLds <= 160 * 1024);
+// workgroups per CU at the class's occupancy target (4 waves per SIMD for the
+// classes that reach it)
+constexpr int kWindGroupsPerCu = (kWavesPerEu == 2 ? 8 : 16) / kWindWaves;
+constexpr bool kWindLds = (kWavesPerEu == 4 || kWavesPerEu == 2) && kWindGroupsPerCu * kWindGroupLds <= 160 * 1024;
 
 __global__ __launch_bounds__(FTL_WAVE * kWindWaves) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
 void ftl_plain_wind_kernel(FtlStep arg) {
@@ -78,13 +88,15 @@ void ftl_plain_wind_kernel(FtlStep arg) {
   __shared__ float4 lattice[kWindLds ? kLatticeF4 : 1];
   __shared__ ftl_tile_impl::TileLds<CL::M, CL::G, CL::GU> sh[kWindLds ? kWindWaves : 1];
   if constexpr (kWindLds) {
-    for (int i = (int)threadIdx.x; i < kLatticeF4; i += FTL_WAVE * kWindWaves) lattice[i] = A.wind[i];
+    for (int i = (int)threadIdx.x; i < kLatticeF4; i += FTL_WAVE * kWindWaves)
+      lattice[i] = A.wind[kWindHalf ? 2 * i : i];
     __syncthreads();
     constexpr int spt = FTL_WAVE / CL::G;
     const int ntiles = (A.nstrands + spt - 1) / spt;
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / FTL_WAVE);
     const int t = (int)blockIdx.x * kWindWaves + w;
-    if (t < ntiles) ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead, true>(A, t * spt, sh[w], lattice);
+    if (t < ntiles)
+      ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead, kWindHalf ? 2 : 1>(A, t * spt, sh[w], lattice);
   }
 }
 

@@ -363,9 +363,10 @@ struct TileLds : TileLens<M, G == 1> {
 
 // PD: rows of the load phase whose loads are in flight ahead of the row being
 // integrated (each costs 7 VGPRs; the forms with a high occupancy target use 1).
-// LW: the wind gathers read wl, the groom's wind lattice copied to LDS by the
-// workgroup, instead of the lattice in global memory.
-template <int M, int G, bool GUARD, int PD_ = FTL_LOAD_AHEAD, bool LW = false>
+// LW: where the wind gathers read the groom's wind lattice: 0 global memory,
+// 1 wl = the whole lattice copied to LDS by the workgroup, 2 wl = its half
+// lattice (first float4 of each node) in LDS.
+template <int M, int G, bool GUARD, int PD_ = FTL_LOAD_AHEAD, int LW = 0>
 __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<M, G, GUARD>& sh,
                                          const float4* wl = nullptr) {
   float4* __restrict__ stage = sh.stage;
@@ -439,7 +440,14 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     const int cp = (r % G) * M + c * C + j;
     const float4 x = xr[i % (PD + 1)], v = vr[i % (PD + 1)];
     WindTaps wt;
-    if (wind) wt = LW ? wind_fetch_from(wl, W, f3(x.x, x.y, x.z)) : wind_fetch(W, f3(x.x, x.y, x.z));
+    if (wind) {
+      if constexpr (LW == 2)
+        wt = wind_fetch_half(wl, W, f3(x.x, x.y, x.z));
+      else if constexpr (LW == 1)
+        wt = wind_fetch_from(wl, W, f3(x.x, x.y, x.z));
+      else
+        wt = wind_fetch(W, f3(x.x, x.y, x.z));
+    }
     __builtin_amdgcn_sched_barrier(0);
     if (i + PD < M) load_row(i + PD);
     __builtin_amdgcn_sched_barrier(0);

@@ -152,8 +152,8 @@ def test_wind_and_rotating_emitter(ctx):
 
 @pytest.mark.parametrize("nstrands,npoints,lanes", [(150100, 32, 2), (140000, 20, 1), (150100, 16, 2)])
 def test_wind_lds_workgroups(ctx, nstrands, npoints, lanes):
-    """Plain launches with wind run workgroups of eight tiles that share an
-    LDS copy of the lattice (the 2-lane 32- and 16-point classes and the
+    """Plain launches with wind run workgroups of four tiles that share an
+    LDS copy of the half lattice (the 2-lane 32- and 16-point classes and the
     guarded 32-point class).  Hundreds of workgroups, the last one partial
     and its last tile partial."""
     pos, vel, _ = ba.sphere_groom(nstrands, npoints, radius=1.0, length=0.3, seed=5)
