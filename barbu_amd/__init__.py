@@ -84,8 +84,10 @@ class Context:
         path = cache_path.encode() if cache_path else None
         check(lib().ftl_autotune(self._h, path, 1 if force else 0), "ftl_autotune")
 
-    def lanes_for(self, npoints):
-        return lib().ftl_lanes_for(self._h, int(npoints))
+    def lanes_for(self, npoints, wind=False):
+        """Lanes per strand the library picks for npoints (plain steps with
+        wind take their own table)."""
+        return (lib().ftl_lanes_for_wind if wind else lib().ftl_lanes_for)(self._h, int(npoints))
 
     def close(self):
         if self._h:

@@ -176,7 +176,7 @@ extern "C" ftl_status ftl_init(int device, ftl_ctx** out) {
   ftl_ctx* c = new ftl_ctx();
   c->device = device;
   c->ncu = prop.multiProcessorCount;
-  for (int i = 0; i <= FTL_MAX_POINTS; ++i) c->lanes[i] = 0;
+  for (int i = 0; i <= FTL_MAX_POINTS; ++i) c->lanes[i] = c->lanes_wind[i] = 0;
   *out = c;
   return FTL_OK;
 }
@@ -211,9 +211,10 @@ static int default_lanes(int npoints) {
   }
 }
 
-int ftl_class_for(ftl_ctx* ctx, int npoints, int lanes, int mmax) {
+int ftl_class_for(ftl_ctx* ctx, int npoints, int lanes, int mmax, bool wind) {
   int G = lanes;
-  if (G == 0 && ctx && npoints >= 0 && npoints <= FTL_MAX_POINTS) G = ctx->lanes[npoints];
+  if (G == 0 && ctx && npoints >= 0 && npoints <= FTL_MAX_POINTS)
+    G = (wind && ctx->lanes_wind[npoints]) ? ctx->lanes_wind[npoints] : ctx->lanes[npoints];
   if (G == 0) G = default_lanes(npoints);
   // exact class with the requested lanes
   for (int c = 0; c < FTL_NCLASSES; ++c) {
@@ -237,6 +238,11 @@ int ftl_class_for(ftl_ctx* ctx, int npoints, int lanes, int mmax) {
 
 extern "C" int ftl_lanes_for(ftl_ctx* ctx, int npoints) {
   const int c = ftl_class_for(ctx, npoints, 0, FTL_MAX_POINTS);
+  return c < 0 ? 0 : kFtlClasses[c].g;
+}
+
+extern "C" int ftl_lanes_for_wind(ftl_ctx* ctx, int npoints) {
+  const int c = ftl_class_for(ctx, npoints, 0, FTL_MAX_POINTS, true);
   return c < 0 ? 0 : kFtlClasses[c].g;
 }
 
@@ -504,7 +510,7 @@ extern "C" ftl_status ftl_groom_step(ftl_groom* g, float dt, void* stream) {
   FtlStep A;
   ftl_status st = ftl_make_step(g, dt, &A);
   if (st != FTL_OK) return st;
-  const int cls = ftl_class_for(g->ctx, g->npoints, g->lanes, FTL_MAX_POINTS);
+  const int cls = ftl_class_for(g->ctx, g->npoints, g->lanes, FTL_MAX_POINTS, A.wind_grid != 0);
   if (cls < 0) return ftl_fail(FTL_ERR_UNSUPPORTED, "no launch class for npoints=%d", g->npoints);
   hipError_t e;
   if ((e = hipSetDevice(g->ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");

@@ -41,19 +41,26 @@ static bool load_cache(ftl_ctx* ctx, const char* path) {
       if (v > 0.0f && v < 1e3f) ctx->cls_ns_cp[c] = v;
     }
   }
-  const size_t at = s.find("\"lanes\"");
-  if (at == std::string::npos) return false;
-  int found = 0;
-  for (int n : kTunedPoints) {
-    std::snprintf(key, sizeof key, "\"%d\": ", n);
-    const size_t k = s.find(key, at);
-    if (k == std::string::npos) continue;
-    const int g = std::atoi(s.c_str() + k + std::strlen(key));
-    if (g == 1 || g == 2 || g == 4) {
-      ctx->lanes[n] = g;
-      ++found;
+  // "lanes": {...} and the optional "lanes_wind": {...}
+  auto read_table = [&](const char* name, int* out) {
+    const size_t at = s.find(name);
+    if (at == std::string::npos) return 0;
+    const size_t end = s.find('}', at);
+    int found = 0;
+    for (int n : kTunedPoints) {
+      std::snprintf(key, sizeof key, "\"%d\": ", n);
+      const size_t k = s.find(key, at);
+      if (k == std::string::npos || k > end) continue;
+      const int g = std::atoi(s.c_str() + k + std::strlen(key));
+      if (g == 1 || g == 2 || g == 4) {
+        out[n] = g;
+        ++found;
+      }
     }
-  }
+    return found;
+  };
+  const int found = read_table("\"lanes\"", ctx->lanes);
+  read_table("\"lanes_wind\"", ctx->lanes_wind);
   return found > 0;
 }
 
@@ -65,6 +72,13 @@ static void save_cache(ftl_ctx* ctx, const char* path, const std::vector<std::st
   for (int n : kTunedPoints) {
     if (!ctx->lanes[n]) continue;
     f << (first ? "" : ",") << "\n    \"" << n << "\": " << ctx->lanes[n];
+    first = false;
+  }
+  f << "\n  },\n  \"lanes_wind\": {";
+  first = true;
+  for (int n : kTunedPoints) {
+    if (!ctx->lanes_wind[n]) continue;
+    f << (first ? "" : ",") << "\n    \"" << n << "\": " << ctx->lanes_wind[n];
     first = false;
   }
   f << "\n  },\n  \"class_ns_per_cp\": {";
@@ -109,18 +123,20 @@ extern "C" ftl_status ftl_autotune(ftl_ctx* ctx, const char* cache_path, int for
     ftl_groom_desc gd{S, N, pos.data(), vel.data(), nullptr};
     ftl_groom* g = nullptr;
     if ((st = ftl_groom_create(ctx, &gd, &g)) != FTL_OK) break;
+    const float sph[8] = {0.f, 0.f, 0.f, 1.f, 0.f, 1.2f, 0.f, 0.3f};
+    ftl_groom_set_colliders(g, sph, 2);
+    // Without wind (batched and served steps, plain steps without wind), then
+    // with wind: plain steps with wind run the LDS-lattice form, and the
+    // gathers change the balance between the load phase and the sweeps.
+    for (int wind = 0; wind <= 1 && st == FTL_OK; ++wind) {
     ftl_params p;
     ftl_params_reference(&p);
     p.force_scale = 1.0f;
     p.length_scale = 1.0f;
     p.damping = 0.9f;
-    // Wind on, as in a typical scene: its gathers change the balance between
-    // the load phase and the sweeps.
-    p.wind_amplitude = 2.0f;
+    p.wind_amplitude = wind ? 2.0f : 0.0f;
     p.wind_frequency = 2.0f;
     ftl_groom_set_params(g, &p);
-    const float sph[8] = {0.f, 0.f, 0.f, 1.f, 0.f, 1.2f, 0.f, 0.3f};
-    ftl_groom_set_colliders(g, sph, 2);
     int best_g = 0;
     float best_t = 1e30f;
     for (int G = 1; G <= 4; G *= 2) {
@@ -147,18 +163,22 @@ extern "C" ftl_status ftl_autotune(ftl_ctx* ctx, const char* cache_path, int for
       float ms = 0;
       (void)hipEventElapsedTime(&ms, t0, t1);
       const float us = 1000.0f * ms / reps;
-      char note[96];
-      std::snprintf(note, sizeof note, "{\"npoints\": %d, \"lanes\": %d, \"us\": %.2f}", N, G, us);
+      char note[112];
+      std::snprintf(note, sizeof note, "{\"npoints\": %d, \"lanes\": %d, \"wind\": %d, \"us\": %.2f}", N, G, wind,
+                    us);
       notes.push_back(note);
-      ctx->cls_ns_cp[cls] = 1000.0f * us / ((float)S * (float)N);
+      // per-class cost, for ordering batched frames (which run without the
+      // LDS-lattice form): the timing without wind
+      if (!wind) ctx->cls_ns_cp[cls] = 1000.0f * us / ((float)S * (float)N);
       if (us < best_t) {
         best_t = us;
         best_g = G;
       }
     }
+    if (st == FTL_OK && best_g) (wind ? ctx->lanes_wind : ctx->lanes)[N] = best_g;
+    }
     ftl_groom_destroy(g);
     if (st != FTL_OK) break;
-    if (best_g) ctx->lanes[N] = best_g;
   }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);

@@ -14,7 +14,8 @@
 struct ftl_ctx {
   int device = 0;
   int ncu = 0;
-  int lanes[FTL_MAX_POINTS + 1];  // tuned lanes per strand by point count (0 = default rule)
+  int lanes[FTL_MAX_POINTS + 1];       // tuned lanes per strand by point count (0 = default rule)
+  int lanes_wind[FTL_MAX_POINTS + 1];  // the same for plain launches with wind (0 = as lanes)
   float cls_ns_cp[FTL_NCLASSES] = {};  // measured ns per control point of each class (0 = not measured)
   std::mutex mu;
   uint64_t bytes = 0;  // device memory held by the context's objects
@@ -84,7 +85,9 @@ void ftl_dev_free(ftl_ctx* ctx, void* p, size_t bytes);
 ftl_status ftl_make_step(ftl_groom* g, float dt, FtlStep* A);
 // Launch class for npoints: requested lanes (0 = tuned/default), at most mmax
 // points per lane.  -1 if none.
-int ftl_class_for(ftl_ctx* ctx, int npoints, int lanes, int mmax);
+// Launch class of a groom: lanes = the groom's setting (0 = tuned); wind: a
+// plain launch with wind (it runs the LDS-lattice form, tuned separately).
+int ftl_class_for(ftl_ctx* ctx, int npoints, int lanes, int mmax, bool wind = false);
 
 // Batched steps (ftl_batch_api.cpp), shared by the batch objects and the
 // serving loop.

@@ -89,7 +89,7 @@ def bench_single(ba, hipm, steps, warmup, S=262144, N=32, lanes=None):
     gpu_ms = e0.elapsed_ms(e1)
     p, v = g.download()
     assert np.isfinite(p).all() and np.isfinite(v).all(), "non-finite state after the benchmark"
-    lanes = lanes or ctx.lanes_for(N)
+    lanes = lanes or ctx.lanes_for(N, wind=True)
     g.close()
     copy_gbs = copy_reference(ba, hipm, ctx, S * N * 32)
     ctx.close()

@@ -249,12 +249,17 @@ ftl_status ftl_diag_free(ftl_ctx* ctx, void* p, size_t bytes);
 /* -- launch-shape tuning ---------------------------------------------------- */
 
 /* Times the candidate lanes-per-strand for every point count on the current
-   device and stores the choice in the context (used when lanes = 0).  If
-   cache_path is not NULL the table is read from it when present and written
-   to it after tuning (JSON). */
+   device, without and with wind, and stores both choices in the context (used
+   when lanes = 0; plain steps with wind run a different kernel form, so they
+   take the second).  If cache_path is not NULL the tables are read from it
+   when present and written to it after tuning (JSON). */
 ftl_status ftl_autotune(ftl_ctx* ctx, const char* cache_path, int force);
-/* The chosen lanes per strand for npoints (1, 2 or 4). */
+/* The chosen lanes per strand for npoints (1, 2 or 4): batched and served
+   steps, and plain steps without wind. */
 int ftl_lanes_for(ftl_ctx* ctx, int npoints);
+/* The same for plain steps with wind.  No counterpart in the reference (its
+   workgroup shape is fixed). */
+int ftl_lanes_for_wind(ftl_ctx* ctx, int npoints);
 
 #ifdef __cplusplus
 }
