@@ -1,6 +1,6 @@
 // Launch classes of the FTL step: (points per lane M, lanes per strand G,
-// guarded).  Exact classes need npoints == M * G; guarded classes (G = 1)
-// accept any npoints <= M.  The class id indexes the per-class kernels and the
+// guarded).  Exact classes need npoints == M * G; guarded classes accept any
+// npoints <= M (G = 1) or (G - 1) * M < npoints <= G * M (G = 2).  The class id indexes the per-class kernels and the
 // `cls` field of batched tiles.
 #pragma once
 
@@ -27,9 +27,12 @@
   X(13, 32, 1, true, 2, 4)    \
   X(14, 64, 1, true, 1, 8)    \
   X(15, 4, 2, false, 7, 4)     \
-  X(16, 48, 1, true, 1, 8)
+  X(16, 48, 1, true, 1, 8)     \
+  X(17, 8, 2, true, 4, 8)      \
+  X(18, 16, 2, true, 2, 8)     \
+  X(19, 32, 2, true, 2, 8)
 
-#define FTL_NCLASSES 17
+#define FTL_NCLASSES 20
 
 struct FtlClassInfo {
   int m, g, guarded, waves, chunk;

@@ -276,14 +276,20 @@ __device__ __forceinline__ void collide_pred(const Spheres& S, float3& p, float3
 // the pipeline drains).  State changes are selects, so no value is live across
 // the two sides of a divergent branch.  FINS: some lane may be in its last
 // iteration (the last G time steps); the earlier steps skip the velocity work.
-template <int M, int G, bool FINS>
+// GUARD (strands of N = NS points, the last lane of a strand holding fewer
+// than M): tip is that lane's slot of the strand's last point (-1 on the other
+// lanes).  The slots after it hold copies of the last input point; they are
+// swept (finite, never stored) but take no velocity work, and the tip itself
+// is finished at its own slot in the last iteration (its velocity is its own
+// correction, as for the last lane of a full strand).
+template <int M, int G, bool FINS, bool GUARD = false>
 __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, const Spheres& S, float3 (&p)[M],
                                            const LenRegs<M>& l,
                                            float3 lead, int g, bool active, bool fin, int gfin,
                                            const float4* __restrict__ pp, float4* __restrict__ stage, int strand0,
                                            int lane, float3 u0, float3& last_pre, float3& cfirst, float3& clast,
-                                           unsigned cm) {
-  constexpr int C = Tile<M, G, false>::C;
+                                           unsigned cm, int NS = M * G, int tip = -1, bool* tbad = nullptr) {
+  constexpr int C = Tile<M, G, GUARD>::C;
   float sg = A.sigma;
   asm volatile("" : "+v"(sg));
   const float dl = V.delta;
@@ -310,26 +316,45 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, c
       const float3 c = sub3(pn, q);
       if (j == 0) cfirst = sel3(fin, c, cfirst);
       if (j >= 1) {
-        const bool root = (j - 1 == 0) && (g == 0);
-        float3 pm = p[j - 1];
+        const int m = j - 1;
+        const bool root = (m == 0) && (g == 0);
+        // GUARD: m is a real point before the tip (or any point of a full lane)
+        const bool real = !GUARD || tip < 0 || m < tip;
+        const bool skip = GUARD && fin && m == tip;  // finished at its own slot
+        float3 pm = p[m];
         float3 u = mul3(c, dl);
-        if (j - 1 == 0) u = sel3(g == 0, u0, u);
-        collide_pred(S, pm, u, active && !root, fin, cm);
-        p[j - 1] = pm;
+        if (m == 0) u = sel3(g == 0, u0, u);
+        collide_pred(S, pm, u, active && !root && !skip, fin, cm);
+        p[m] = pm;
         // Final velocities go through the LDS stage; a completed chunk of the
         // finishing lanes leaves coalesced.  The last chunk waits for the
         // lane's last point (after the pipeline drains).
-        const int m = j - 1;
         if constexpr (kStagedVel<M, G>) {
-          if (fin) {
+          if (fin && real) {
             const float3 v = out_velocity(V, pp, m, pm, u);
             stage[slot<C>(lane, m % C)] = make_float4(v.x, v.y, v.z, 0.0f);
           }
           if (m % C == C - 1 && m / C < M / C - 1)
-            flush_chunk<M, G, false>(A.vel, strand0, A.nstrands, M * G, m / C, stage, lane, gfin);
-        } else if (fin && strand0 + lane / G < A.nstrands) {
+            flush_chunk<M, G, GUARD>(A.vel, strand0, A.nstrands, NS, m / C, stage, lane, gfin);
+        } else if (fin && real && strand0 + lane / G < A.nstrands) {
           const float3 v = out_velocity(V, pp, m, pm, u);
-          A.vel[(size_t)(strand0 + lane / G) * (M * G) + g * M + m] = make_float4(v.x, v.y, v.z, 0.0f);
+          A.vel[(size_t)(strand0 + lane / G) * NS + g * M + m] = make_float4(v.x, v.y, v.z, 0.0f);
+        }
+      }
+      if constexpr (GUARD) {
+        if (fin && j == tip) {
+          // the strand's last point: its own correction, collided and reflected
+          float3 pt = pn, ut = c;
+          collide_pred(S, pt, ut, true, true, cm);
+          p[j] = pt;
+          const float3 v = out_velocity(V, pp, j, pt, ut);
+          if constexpr (kStagedVel<M, G>) {
+            stage[slot<C>(lane, j % C)] = make_float4(v.x, v.y, v.z, 0.0f);
+          } else if (strand0 + lane / G < A.nstrands) {
+            A.vel[(size_t)(strand0 + lane / G) * NS + g * M + j] = make_float4(v.x, v.y, v.z, 0.0f);
+          }
+          *tbad = !(isfinite(pt.x) && isfinite(pt.y) && isfinite(pt.z) && isfinite(v.x) && isfinite(v.y) &&
+                    isfinite(v.z));
         }
       }
       if (j == M - 1) clast = sel3(fin, c, clast);
@@ -378,7 +403,10 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
   const int lane = tid & (FTL_WAVE - 1);
   const int g = (G == 1) ? 0 : (lane % G);
   const int N = GUARD ? A.npoints : M * G;
-  const int nl = GUARD ? A.npoints : M;  // points held by this lane (G == 1 when GUARD)
+  // points held by this lane: with G > 1 and GUARD the strand's last lane
+  // holds N - (G - 1) * M of its M slots (the guarded pipelined classes take
+  // (G - 1) * M < N <= G * M)
+  const int nl = GUARD ? (G == 1 ? A.npoints : min(max(A.npoints - g * M, 0), M)) : M;
   const int nstr = A.nstrands;
   const int strand = strand0 + lane / G;
   const bool valid = strand < nstr;
@@ -532,42 +560,53 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     }
   } else {
     // No lane is in its last iteration before time step K - 1.
+    // GUARD: the last lane of a strand finishes its tip (slot nl - 1) in its
+    // last sweep; the other lanes keep the post-drain path below.
+    const int tip = (GUARD && g == G - 1) ? nl - 1 : -1;
+    bool tbad = false;
     for (int t = 0; t < K - 1; ++t) {
       const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
                            __shfl_up(last_pre.z, 1, G));
       const float3 lead = sel3(g == 0, p[0], li);
-      sweep_pipe<M, G, false>(A, V, S, p, l, lead, g, t >= g, false, -1, pp, stage, strand0, lane, u0, last_pre,
-                              cfirst, clast, cm);
+      sweep_pipe<M, G, false, GUARD>(A, V, S, p, l, lead, g, t >= g, false, -1, pp, stage, strand0, lane, u0,
+                                     last_pre, cfirst, clast, cm, N, tip, &tbad);
     }
     for (int t = K - 1; t < K + G - 1; ++t) {
       const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
                            __shfl_up(last_pre.z, 1, G));
       const float3 lead = sel3(g == 0, p[0], li);
       const int it = t - g;
-      sweep_pipe<M, G, true>(A, V, S, p, l, lead, g, it >= 0 && it < K, it == K - 1, t - (K - 1), pp, stage,
-                             strand0, lane, u0, last_pre, cfirst, clast, cm);
+      sweep_pipe<M, G, true, GUARD>(A, V, S, p, l, lead, g, it >= 0 && it < K, it == K - 1, t - (K - 1), pp,
+                                    stage, strand0, lane, u0, last_pre, cfirst, clast, cm, N, tip, &tbad);
     }
     // The lane's last point: its velocity needs the next lane's correction.
     const float3 cn = f3(__shfl_down(cfirst.x, 1, G), __shfl_down(cfirst.y, 1, G),
                          __shfl_down(cfirst.z, 1, G));
     u = sel3(g == G - 1, clast, mul3(cn, V.delta));
     float3 pm = p[M - 1];
-    collide<true>(S, pm, u, cm);
+    const bool own = !GUARD || tip < 0;  // GUARD: the tip lane is done
+    if constexpr (GUARD)
+      collide_pred(S, pm, u, own, true, cm);
+    else
+      collide<true>(S, pm, u, cm);
     p[M - 1] = pm;
-    const float3 v = out_velocity(V, pp, M - 1, pm, u);
+    // (GUARD: the tip lane's slot M - 1 may lie past the strand; it reads the
+    // strand's first point instead, and the result is dropped)
+    const float3 v = out_velocity(V, pp, own ? M - 1 : 0, pm, u);
     if constexpr (kStagedVel<M, G>) {
-      stage[slot<C>(lane, (M - 1) % C)] = make_float4(v.x, v.y, v.z, 0.0f);
+      if (own) stage[slot<C>(lane, (M - 1) % C)] = make_float4(v.x, v.y, v.z, 0.0f);
       flush_chunk<M, G, GUARD>(A.vel, strand0, nstr, N, M / C - 1, stage, lane);
-    } else if (valid) {
+    } else if (valid && own) {
       A.vel[(size_t)strand * N + g * M + (M - 1)] = make_float4(v.x, v.y, v.z, 0.0f);
     }
-    bad = !(isfinite(pm.x) && isfinite(pm.y) && isfinite(pm.z) && isfinite(v.x) && isfinite(v.y) &&
-            isfinite(v.z));
+    bad = own ? !(isfinite(pm.x) && isfinite(pm.y) && isfinite(pm.z) && isfinite(v.x) && isfinite(v.y) &&
+                  isfinite(v.z))
+              : tbad;
   }
   // positions, after every velocity (DFTL reads the step's input positions)
 #pragma unroll
   for (int c = 0; c < M / C; ++c) {
-    if (GUARD && c * C >= nl) continue;
+    if (GUARD && G == 1 && c * C >= nl) continue;
 #pragma unroll
     for (int j = 0; j < C; ++j) {
       const float3 q = p[c * C + j];

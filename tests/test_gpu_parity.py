@@ -97,7 +97,7 @@ def test_reference_preset(ctx):
 
 
 CLASSES = [(4, 1), (8, 1), (8, 2), (16, 1), (16, 2), (16, 4), (32, 1), (32, 2), (32, 4), (64, 2), (64, 4), (64, 1)]
-GUARDED = [2, 3, 5, 7, 12, 20, 33, 40, 48, 49, 63]
+GUARDED = [2, 3, 5, 7, 9, 12, 15, 17, 20, 31, 33, 40, 48, 49, 63]
 
 
 @pytest.mark.parametrize("npoints,lanes", CLASSES)
@@ -108,11 +108,15 @@ def test_launch_classes(ctx, npoints, lanes):
     run_compare(ctx, pos, vel, params, spheres, 3, lanes=lanes, label="N=%d G=%d" % (npoints, lanes))
 
 
+@pytest.mark.parametrize("lanes", [0, 1])
 @pytest.mark.parametrize("npoints", GUARDED)
-def test_guarded_point_counts(ctx, npoints):
+def test_guarded_point_counts(ctx, npoints, lanes):
+    """Ragged point counts: by default the two-lane guarded classes (the last
+    lane of a strand partly filled) above 8 points; with one lane asked for,
+    the one-lane guarded classes."""
     pos, vel, _ = ba.sphere_groom(333, npoints, radius=1.0, length=0.25, seed=npoints)
     params = ba.make_params(force_scale=1.0, damping=0.85, length_scale=1.2)
-    run_compare(ctx, pos, vel, params, [(0, 0, 0, 1)], 3, label="N=%d" % npoints)
+    run_compare(ctx, pos, vel, params, [(0, 0, 0, 1)], 3, lanes=lanes, label="N=%d lanes=%d" % (npoints, lanes))
 
 
 @pytest.mark.parametrize("npoints", [3, 5, 8, 20, 32, 48])
@@ -150,12 +154,14 @@ def test_wind_and_rotating_emitter(ctx):
                 times=lambda k: (k + 1) * float(DT), label="wind")
 
 
-@pytest.mark.parametrize("nstrands,npoints,lanes", [(150100, 32, 2), (140000, 20, 1), (150100, 16, 2)])
+@pytest.mark.parametrize("nstrands,npoints,lanes",
+                         [(150100, 32, 2), (140000, 20, 1), (150100, 16, 2), (140000, 20, 0), (100100, 40, 0)])
 def test_wind_lds_workgroups(ctx, nstrands, npoints, lanes):
     """Plain launches with wind run workgroups of four tiles that share an
-    LDS copy of the half lattice (the 2-lane 32- and 16-point classes and the
-    guarded 32-point class).  Hundreds of workgroups, the last one partial
-    and its last tile partial."""
+    LDS copy of the half lattice (the 2-lane 32- and 16-point classes, the
+    one-lane guarded 32-point class and the two-lane guarded 16- and 32-point
+    classes).  Hundreds of workgroups, the last one partial and its last tile
+    partial."""
     pos, vel, _ = ba.sphere_groom(nstrands, npoints, radius=1.0, length=0.3, seed=5)
     params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.45, wind_amplitude=2.0,
                             wind_frequency=2.0, wind_scroll=(0.5, 0.0, 0.25), wind_seed=7)
