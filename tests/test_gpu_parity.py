@@ -137,6 +137,18 @@ def test_dftl_mode_with_clamp(ctx):
     run_compare(ctx, pos, vel, params, [(0, 0, 0, 1), (0.5, 0.8, 0.0, 0.4)], 5, label="dftl")
 
 
+@pytest.mark.parametrize("npoints", [12, 20, 40, 63])
+def test_dftl_mode_ragged(ctx, npoints):
+    """DFTL velocities (they re-read the step's input positions) on ragged
+    strands: the two-lane guarded classes read only the strand's own points,
+    the last one included."""
+    pos, vel, _ = ba.sphere_groom(3000, npoints, radius=1.0, length=0.3, seed=60 + npoints)
+    vel[:, 1:, :3] = np.random.default_rng(npoints).normal(scale=3.0, size=vel[:, 1:, :3].shape)
+    params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.2,
+                            velocity_mode=ba.FTL_VELOCITY_DFTL, max_velocity=2.0)
+    run_compare(ctx, pos, vel, params, [(0, 0, 0, 1), (0.5, 0.8, 0.0, 0.4)], 3, label="dftl N=%d" % npoints)
+
+
 def test_wind_and_rotating_emitter(ctx):
     """C2 features at small size: value-noise wind, emitter rotating about y at
     1 rad/s (rest roots), two colliders (one orbiting)."""
