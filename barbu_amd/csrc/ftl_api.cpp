@@ -227,13 +227,18 @@ int ftl_class_for(ftl_ctx* ctx, int npoints, int lanes, int mmax, bool wind) {
       const FtlClassInfo& k = kFtlClasses[c];
       if (!k.guarded && k.g == g && k.m * k.g == npoints && k.m <= mmax) return c;
     }
-  // guarded, two lanes per strand (unless one lane was asked for): the class
-  // whose first lane the strand fills
-  if (lanes != 1)
+  // guarded, two lanes per strand (unless one lane was asked for): the
+  // smallest class whose first lane the strand fills (the least padding)
+  if (lanes != 1) {
+    int best2 = -1;
     for (int c = 0; c < FTL_NCLASSES; ++c) {
       const FtlClassInfo& k = kFtlClasses[c];
-      if (k.guarded && k.g > 1 && (k.g - 1) * k.m < npoints && npoints <= k.g * k.m && k.m <= mmax) return c;
+      if (k.guarded && k.g > 1 && (k.g - 1) * k.m < npoints && npoints <= k.g * k.m && k.m <= mmax &&
+          (best2 < 0 || k.m < kFtlClasses[best2].m))
+        best2 = c;
     }
+    if (best2 >= 0) return best2;
+  }
   // guarded, one lane per strand: the smallest that holds the strand
   int best = -1;
   for (int c = 0; c < FTL_NCLASSES; ++c) {

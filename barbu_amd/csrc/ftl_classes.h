@@ -30,9 +30,11 @@
   X(16, 48, 1, true, 1, 8)     \
   X(17, 8, 2, true, 4, 8)      \
   X(18, 16, 2, true, 2, 8)     \
-  X(19, 32, 2, true, 2, 8)
+  X(19, 32, 2, true, 2, 8)     \
+  X(20, 12, 2, true, 4, 4)     \
+  X(21, 24, 2, true, 3, 8)
 
-#define FTL_NCLASSES 20
+#define FTL_NCLASSES 22
 
 struct FtlClassInfo {
   int m, g, guarded, waves, chunk;
