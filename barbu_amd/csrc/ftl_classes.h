@@ -1,6 +1,6 @@
 // Launch classes of the FTL step: (points per lane M, lanes per strand G,
 // guarded).  Exact classes need npoints == M * G; guarded classes accept any
-// npoints <= M (G = 1) or (G - 1) * M < npoints <= G * M (G = 2).  The class id indexes the per-class kernels and the
+// npoints <= M (G = 1) or (G - 1) * M < npoints <= G * M (G = 2 or 4).  The class id indexes the per-class kernels and the
 // `cls` field of batched tiles.
 #pragma once
 
@@ -32,9 +32,10 @@
   X(18, 16, 2, true, 2, 8)     \
   X(19, 32, 2, true, 2, 8)     \
   X(20, 12, 2, true, 4, 4)     \
-  X(21, 24, 2, true, 3, 8)
+  X(21, 24, 2, true, 3, 8)     \
+  X(22, 16, 4, true, 3, 8)
 
-#define FTL_NCLASSES 22
+#define FTL_NCLASSES 23
 
 struct FtlClassInfo {
   int m, g, guarded, waves, chunk;

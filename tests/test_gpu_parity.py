@@ -97,7 +97,7 @@ def test_reference_preset(ctx):
 
 
 CLASSES = [(4, 1), (8, 1), (8, 2), (16, 1), (16, 2), (16, 4), (32, 1), (32, 2), (32, 4), (64, 2), (64, 4), (64, 1)]
-GUARDED = [2, 3, 5, 7, 9, 12, 15, 17, 20, 31, 33, 40, 48, 49, 63]
+GUARDED = [2, 3, 5, 7, 9, 12, 15, 17, 20, 31, 33, 40, 48, 49, 55, 63]
 
 
 @pytest.mark.parametrize("npoints,lanes", CLASSES)
