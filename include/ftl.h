@@ -180,7 +180,9 @@ ftl_status ftl_groom_info(ftl_groom* g, int* nstrands, int* npoints);
 ftl_status ftl_groom_status(ftl_groom* g, int* nonfinite);
 
 /* Launch shape of plain steps: lanes per strand (1, 2 or 4; 0 = automatic).
-   Results do not depend on it. */
+   A point count that no class of that shape holds exactly runs a guarded
+   class: one lane per strand if 1 was asked for, else the pipelined guarded
+   class with the fewest points per lane.  Results do not depend on it. */
 ftl_status ftl_groom_set_lanes(ftl_groom* g, int lanes_per_strand);
 
 /* -- dual-quaternion skinned roots (FTL_ROOT_SKINNED) ----------------------- */
