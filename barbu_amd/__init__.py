@@ -9,7 +9,8 @@ import ctypes as C
 import numpy as np
 
 from . import _ffi
-from ._ffi import (FTL_ROOT_MODEL, FTL_ROOT_REST, FTL_ROOT_SKINNED, FTL_VELOCITY_DFTL,
+from ._ffi import (FTL_ROOT_MODEL, FTL_ROOT_REST, FTL_ROOT_SKINNED, FTL_SKIN_DQ, FTL_SKIN_LBS, FTL_NO_JOINT,
+                   FTL_VELOCITY_DFTL,
                    FTL_VELOCITY_SHIPPED, FTL_MAX_COLLIDERS, FTL_MAX_POINTS, FTL_WIND_GRID,
                    FTL_OK, FTL_ERR_INVALID_ARGUMENT, FTL_ERR_DEVICE, FTL_ERR_OUT_OF_MEMORY, FTL_ERR_BUDGET,
                    FTL_ERR_NOT_INITIALIZED, FTL_ERR_NONFINITE, FTL_ERR_UNSUPPORTED, FTL_ERR_BUSY,
@@ -53,6 +54,15 @@ def make_params(**kw):
         else:
             setattr(p, k, v)
     return p
+
+
+def dualquat_from_matrix(m3x4):
+    """The dual quaternion (8 floats: real xyzw, dual xyzw) of a rigid 3x4
+    transform given as its three rows (ftl_dualquat_from_matrix; host only)."""
+    m = _f32(m3x4).reshape(12)
+    out = np.empty(8, np.float32)
+    check(lib().ftl_dualquat_from_matrix(_ptr(m), _ptr(out)), "ftl_dualquat_from_matrix")
+    return out
 
 
 def wind_table(params):
@@ -160,6 +170,11 @@ class Groom:
     def step(self, dt, stream=None):
         check(lib().ftl_groom_step(self._h, float(dt), stream), "ftl_groom_step")
 
+    def step_timed(self, dt, stream, start_event, stop_event):
+        """step() with its kernel bracketed by two hipEvent_t handles (ftl_groom_step_timed)."""
+        check(lib().ftl_groom_step_timed(self._h, float(dt), stream, start_event, stop_event),
+              "ftl_groom_step_timed")
+
     def download(self):
         pos = np.empty((self.nstrands, self.npoints, 4), np.float32)
         vel = np.empty_like(pos)
@@ -186,6 +201,13 @@ class Groom:
         tri = np.ascontiguousarray(tri, np.int32)
         bary = _f32(bary, (self.nstrands, 2))
         check(lib().ftl_groom_bind_skin(self._h, skin.handle, _ptr(tri), _ptr(bary)), "ftl_groom_bind_skin")
+
+    def root_frames(self, normals=True):
+        """Skinned roots and (if normals) unit root normals, float32 [nstrands, 4] each."""
+        r = np.empty((self.nstrands, 4), np.float32)
+        n = np.empty((self.nstrands, 4), np.float32) if normals else None
+        check(lib().ftl_groom_root_frames(self._h, _ptr(r), _ptr(n)), "ftl_groom_root_frames")
+        return (r, n) if normals else r
 
     def close(self):
         if self._h:
@@ -217,9 +239,31 @@ class Skin:
     def handle(self):
         return self._h
 
+    def set_mode(self, mode):
+        """FTL_SKIN_DQ (dual quaternions) or FTL_SKIN_LBS (linear blending)."""
+        check(lib().ftl_skin_set_mode(self._h, int(mode)), "ftl_skin_set_mode")
+
     def set_bones(self, dq):
         dq = _f32(dq).reshape(-1, 8)
         check(lib().ftl_skin_set_bones(self._h, _ptr(dq), int(dq.shape[0])), "ftl_skin_set_bones")
+
+    def set_matrices(self, m3x4):
+        """Skinning matrices, (nbones, 3, 4): the rows of each bone's affine matrix."""
+        m = _f32(m3x4).reshape(-1, 12)
+        check(lib().ftl_skin_set_matrices(self._h, _ptr(m), int(m.shape[0])), "ftl_skin_set_matrices")
+
+    def set_joint_matrices(self, global_pose, inverse_bind=None):
+        """Row-major (nbones, 4, 4) global poses and inverse bind matrices; the
+        library forms skin_i = global_i @ inverse_bind_i (column-major at the C ABI)."""
+        g = np.ascontiguousarray(np.transpose(_f32(global_pose).reshape(-1, 4, 4), (0, 2, 1)))
+        ib = None if inverse_bind is None else np.ascontiguousarray(
+            np.transpose(_f32(inverse_bind).reshape(-1, 4, 4), (0, 2, 1)))
+        check(lib().ftl_skin_set_joint_matrices(self._h, _ptr(g), _ptr(ib), int(g.shape[0])),
+              "ftl_skin_set_joint_matrices")
+
+    def set_rest_normals(self, normals):
+        n = _f32(normals).reshape(self.rest.shape[0], 4)
+        check(lib().ftl_skin_set_rest_normals(self._h, _ptr(n)), "ftl_skin_set_rest_normals")
 
     def update(self, stream=None):
         check(lib().ftl_skin_update(self._h, stream), "ftl_skin_update")
@@ -227,6 +271,11 @@ class Skin:
     def download(self):
         out = np.empty((self.rest.shape[0], 4), np.float32)
         check(lib().ftl_skin_download(self._h, _ptr(out)), "ftl_skin_download")
+        return out
+
+    def download_normals(self):
+        out = np.empty((self.rest.shape[0], 4), np.float32)
+        check(lib().ftl_skin_download_normals(self._h, _ptr(out)), "ftl_skin_download_normals")
         return out
 
     def close(self):
@@ -316,8 +365,10 @@ def set_request(r, dt=None, time=None, model=None, spheres=None):
 class Server:
     """Resident creatures stepped by continuous batching (ftl_server_*)."""
 
-    def __init__(self, ctx, hbm_budget_bytes=0, batch_cap_bytes=0, max_inflight=0):
-        cfg = _ffi.ServerConfig(int(hbm_budget_bytes), int(batch_cap_bytes), int(max_inflight))
+    def __init__(self, ctx, hbm_budget_bytes=0, batch_cap_bytes=0, max_inflight=0, coalesce_us=0,
+                 batch_target_bytes=0):
+        cfg = _ffi.ServerConfig(int(hbm_budget_bytes), int(batch_cap_bytes), int(max_inflight), int(coalesce_us),
+                                int(batch_target_bytes))
         self._h = C.c_void_p()
         self.ctx = ctx
         self.shapes = {}
@@ -379,9 +430,17 @@ class Server:
         return pos, vel
 
     def status(self, cid):
+        """True when the creature is quarantined (a step left non-finite values)."""
         nf = C.c_int(0)
         check(lib().ftl_server_status(self._h, int(cid), C.byref(nf)), "ftl_server_status")
         return bool(nf.value)
+
+    def upload(self, cid, pos, vel):
+        """Replaces the creature's state and lifts its quarantine (ftl_server_upload)."""
+        S, N = self.shapes[cid]
+        pos = _f32(pos, (S, N, 4))
+        vel = _f32(vel, (S, N, 4))
+        check(lib().ftl_server_upload(self._h, int(cid), _ptr(pos), _ptr(vel)), "ftl_server_upload")
 
     def stats(self):
         st = _ffi.ServerStats()

@@ -20,6 +20,9 @@ FTL_VELOCITY_DFTL = 1
 FTL_ROOT_MODEL = 0
 FTL_ROOT_REST = 1
 FTL_ROOT_SKINNED = 2
+FTL_SKIN_DQ = 0
+FTL_SKIN_LBS = 1
+FTL_NO_JOINT = 0xFF
 FTL_MAX_COLLIDERS = 4
 FTL_MAX_POINTS = 64
 FTL_WIND_GRID = 8
@@ -109,6 +112,8 @@ class ServerConfig(C.Structure):
         ("hbm_budget_bytes", C.c_uint64),
         ("batch_cap_bytes", C.c_uint64),
         ("max_inflight", C.c_int),
+        ("coalesce_us", C.c_int),
+        ("batch_target_bytes", C.c_uint64),
     ]
 
 
@@ -139,6 +144,10 @@ class ServerStats(C.Structure):
         ("resident_bytes", C.c_uint64),
         ("resident", C.c_int),
         ("waiting", C.c_int),
+        ("cps_finite", C.c_uint64),
+        ("dropped", C.c_uint64),
+        ("failed_admissions", C.c_uint64),
+        ("quarantined", C.c_int),
     ]
 
 
@@ -165,6 +174,7 @@ SIGNATURES = {
     "ftl_groom_set_rest_roots": (st, [vp, vp]),
     "ftl_groom_set_time": (st, [vp, C.c_double]),
     "ftl_groom_step": (st, [vp, C.c_float, vp]),
+    "ftl_groom_step_timed": (st, [vp, C.c_float, vp, vp, vp]),
     "ftl_groom_device_buffers": (st, [vp, pp, pp]),
     "ftl_groom_download": (st, [vp, vp, vp]),
     "ftl_groom_upload": (st, [vp, vp, vp]),
@@ -176,7 +186,14 @@ SIGNATURES = {
     "ftl_skin_set_bones": (st, [vp, vp, C.c_int]),
     "ftl_skin_update": (st, [vp, vp]),
     "ftl_skin_download": (st, [vp, vp]),
+    "ftl_skin_set_mode": (st, [vp, C.c_int]),
+    "ftl_skin_set_matrices": (st, [vp, vp, C.c_int]),
+    "ftl_skin_set_joint_matrices": (st, [vp, vp, vp, C.c_int]),
+    "ftl_skin_set_rest_normals": (st, [vp, vp]),
+    "ftl_skin_download_normals": (st, [vp, vp]),
+    "ftl_dualquat_from_matrix": (st, [vp, vp]),
     "ftl_groom_bind_skin": (st, [vp, vp, vp, vp]),
+    "ftl_groom_root_frames": (st, [vp, vp, vp]),
     "ftl_batch_step": (st, [vp, vp, C.c_int, vp, vp]),
     "ftl_batch_create": (st, [vp, vp, C.c_int, pp]),
     "ftl_batch_destroy": (None, [vp]),
@@ -205,6 +222,7 @@ SIGNATURES = {
     "ftl_server_drain": (st, [vp, C.c_int]),
     "ftl_server_download": (st, [vp, C.c_int, vp, vp]),
     "ftl_server_status": (st, [vp, C.c_int, ip]),
+    "ftl_server_upload": (st, [vp, C.c_int, vp, vp]),
     "ftl_server_get_stats": (st, [vp, C.POINTER(ServerStats)]),
     "ftl_server_reset_stats": (st, [vp]),
     "ftl_scn_strand_tangents": (st, [vp, C.c_int, C.c_int, vp]),

@@ -112,7 +112,7 @@ extern "C" void ftl_params_reference(ftl_params* p) {
   p->wind_seed = 0;
 }
 
-static ftl_status check_params(const ftl_params* p) {
+ftl_status ftl_check_params(const ftl_params* p) {
   if (!p) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "params is NULL");
   if (p->iterations < 1 || p->iterations > 256)
     return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "iterations must be in [1, 256] (got %d)", p->iterations);
@@ -183,6 +183,11 @@ extern "C" ftl_status ftl_init(int device, ftl_ctx** out) {
 
 extern "C" ftl_status ftl_shutdown(ftl_ctx* ctx) {
   if (!ctx) return FTL_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->scratch_ev) {
+    if (ctx->scratch_ev_recorded) (void)hipEventSynchronize(ctx->scratch_ev);
+    (void)hipEventDestroy(ctx->scratch_ev);
+  }
   if (ctx->scratch_steps) ftl_dev_free(ctx, ctx->scratch_steps, ctx->scratch_steps_bytes);
   if (ctx->scratch_tiles) ftl_dev_free(ctx, ctx->scratch_tiles, ctx->scratch_tiles_bytes);
   if (ctx->live_objects != 0) {
@@ -353,7 +358,7 @@ extern "C" void ftl_groom_destroy(ftl_groom* g) {
 
 extern "C" ftl_status ftl_groom_set_params(ftl_groom* g, const ftl_params* p) {
   if (!g) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom is NULL");
-  ftl_status st = check_params(p);
+  ftl_status st = ftl_check_params(p);
   if (st != FTL_OK) return st;
   g->params = *p;
   return FTL_OK;
@@ -517,7 +522,7 @@ ftl_status ftl_make_step(ftl_groom* g, float dt, FtlStep* A) {
   return FTL_OK;
 }
 
-extern "C" ftl_status ftl_groom_step(ftl_groom* g, float dt, void* stream) {
+static ftl_status groom_step(ftl_groom* g, float dt, void* stream, hipEvent_t start, hipEvent_t stop) {
   if (!g) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom is NULL");
   FtlStep A;
   ftl_status st = ftl_make_step(g, dt, &A);
@@ -526,8 +531,19 @@ extern "C" ftl_status ftl_groom_step(ftl_groom* g, float dt, void* stream) {
   if (cls < 0) return ftl_fail(FTL_ERR_UNSUPPORTED, "no launch class for npoints=%d", g->npoints);
   hipError_t e;
   if ((e = hipSetDevice(g->ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
-  if ((e = ftl_launch_plain(cls, A, (hipStream_t)stream)) != hipSuccess) return ftl_hip_fail(e, "FTL step launch");
+  if ((e = ftl_launch_plain(cls, A, (hipStream_t)stream, start, stop)) != hipSuccess)
+    return ftl_hip_fail(e, "FTL step launch");
   return FTL_OK;
+}
+
+extern "C" ftl_status ftl_groom_step(ftl_groom* g, float dt, void* stream) {
+  return groom_step(g, dt, stream, nullptr, nullptr);
+}
+
+extern "C" ftl_status ftl_groom_step_timed(ftl_groom* g, float dt, void* stream, void* start_event,
+                                           void* stop_event) {
+  if (!start_event || !stop_event) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL event");
+  return groom_step(g, dt, stream, (hipEvent_t)start_event, (hipEvent_t)stop_event);
 }
 
 extern "C" ftl_status ftl_groom_device_buffers(ftl_groom* g, void** pos, void** vel) {
@@ -593,9 +609,15 @@ extern "C" ftl_status ftl_skin_create(ftl_ctx* ctx, const ftl_skin_desc* d, ftl_
     return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "skin sizes out of range");
   if (!d->rest_positions || !d->joints || !d->weights || !d->triangles)
     return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL skin array");
-  for (long i = 0; i < 4L * d->nverts; ++i)
+  bool no_joint = false;
+  for (long i = 0; i < 4L * d->nverts; ++i) {
+    if (d->joints[i] == FTL_NO_JOINT) {
+      no_joint = true;
+      continue;
+    }
     if (d->joints[i] < 0 || d->joints[i] >= d->nbones)
       return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "joint index %d out of range at vertex %ld", d->joints[i], i / 4);
+  }
   for (long i = 0; i < 3L * d->ntris; ++i)
     if (d->triangles[i] < 0 || d->triangles[i] >= d->nverts)
       return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "vertex index out of range in triangle %ld", i / 3);
@@ -606,7 +628,9 @@ extern "C" ftl_status ftl_skin_create(ftl_ctx* ctx, const ftl_skin_desc* d, ftl_
   s->nverts = d->nverts;
   s->ntris = d->ntris;
   s->nbones = d->nbones;
-  const size_t vb = (size_t)d->nverts * 16, tb = (size_t)d->ntris * 16, bb = (size_t)d->nbones * 32;
+  s->has_no_joint = no_joint;
+  const size_t vb = (size_t)d->nverts * 16, tb = (size_t)d->ntris * 16, bb = (size_t)d->nbones * 32,
+               mb = (size_t)d->nbones * 48;
   std::vector<int> tris4((size_t)d->ntris * 4, 0);
   for (int t = 0; t < d->ntris; ++t)
     for (int c = 0; c < 3; ++c) tris4[4 * (size_t)t + c] = d->triangles[3 * (size_t)t + c];
@@ -616,7 +640,8 @@ extern "C" ftl_status ftl_skin_create(ftl_ctx* ctx, const ftl_skin_desc* d, ftl_
       (st = ftl_dev_alloc(ctx, (void**)&s->d_weights, vb, "skin weights")) != FTL_OK ||
       (st = ftl_dev_alloc(ctx, (void**)&s->d_out, vb, "skin out")) != FTL_OK ||
       (st = ftl_dev_alloc(ctx, (void**)&s->d_tris, tb, "skin tris")) != FTL_OK ||
-      (st = ftl_dev_alloc(ctx, (void**)&s->d_dq, bb, "skin bones")) != FTL_OK) {
+      (st = ftl_dev_alloc(ctx, (void**)&s->d_dq, bb, "skin bones")) != FTL_OK ||
+      (st = ftl_dev_alloc(ctx, (void**)&s->d_mats, mb, "skin matrices")) != FTL_OK) {
     {
       std::lock_guard<std::mutex> lk(ctx->mu);
       ctx->live_objects++;
@@ -629,10 +654,15 @@ extern "C" ftl_status ftl_skin_create(ftl_ctx* ctx, const ftl_skin_desc* d, ftl_
             hipMemcpy(s->d_weights, d->weights, vb, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(s->d_out, d->rest_positions, vb, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(s->d_tris, tris4.data(), tb, hipMemcpyHostToDevice) == hipSuccess;
-  // identity dual quaternions until bones are set
+  // identity bones until they are set: dual quaternions (1, 0) and matrices [I | 0]
   s->h_dq.assign((size_t)d->nbones * 8, 0.0f);
-  for (int b = 0; b < d->nbones; ++b) s->h_dq[8 * (size_t)b + 3] = 1.0f;
-  ok = ok && hipMemcpy(s->d_dq, s->h_dq.data(), bb, hipMemcpyHostToDevice) == hipSuccess;
+  s->h_mats.assign((size_t)d->nbones * 12, 0.0f);
+  for (int b = 0; b < d->nbones; ++b) {
+    s->h_dq[8 * (size_t)b + 3] = 1.0f;
+    for (int r = 0; r < 3; ++r) s->h_mats[12 * (size_t)b + 5 * r] = 1.0f;
+  }
+  ok = ok && hipMemcpy(s->d_dq, s->h_dq.data(), bb, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(s->d_mats, s->h_mats.data(), mb, hipMemcpyHostToDevice) == hipSuccess;
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->live_objects++;
@@ -650,18 +680,33 @@ extern "C" void ftl_skin_destroy(ftl_skin* s) {
   ftl_ctx* ctx = s->ctx;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
-  const size_t vb = (size_t)s->nverts * 16, tb = (size_t)s->ntris * 16, bb = (size_t)s->nbones * 32;
+  const size_t vb = (size_t)s->nverts * 16, tb = (size_t)s->ntris * 16, bb = (size_t)s->nbones * 32,
+               mb = (size_t)s->nbones * 48;
   ftl_dev_free(ctx, s->d_rest, vb);
   ftl_dev_free(ctx, s->d_joints, vb);
   ftl_dev_free(ctx, s->d_weights, vb);
   ftl_dev_free(ctx, s->d_out, vb);
   ftl_dev_free(ctx, s->d_tris, tb);
   ftl_dev_free(ctx, s->d_dq, bb);
+  ftl_dev_free(ctx, s->d_mats, mb);
+  ftl_dev_free(ctx, s->d_nrest, vb);
+  ftl_dev_free(ctx, s->d_nout, vb);
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->live_objects--;
   }
   delete s;
+}
+
+extern "C" ftl_status ftl_skin_set_mode(ftl_skin* s, int mode) {
+  if (!s) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "skin is NULL");
+  if (mode != FTL_SKIN_DQ && mode != FTL_SKIN_LBS)
+    return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "unknown skinning mode %d", mode);
+  if (mode == FTL_SKIN_DQ && s->has_no_joint)
+    return ftl_fail(FTL_ERR_INVALID_ARGUMENT,
+                    "dual-quaternion skinning needs a bone for every joint index (FTL_NO_JOINT is linear-blend only)");
+  s->mode = mode;
+  return FTL_OK;
 }
 
 extern "C" ftl_status ftl_skin_set_bones(ftl_skin* s, const float* dq, int nbones) {
@@ -674,19 +719,131 @@ extern "C" ftl_status ftl_skin_set_bones(ftl_skin* s, const float* dq, int nbone
   return FTL_OK;
 }
 
+// The unit quaternion of rotation R (rows R[r][c]), by the largest of
+// w, x, y, z (Shepperd's method: the square root is taken of the largest
+// diagonal combination, so no division by a small number), then the dual part
+// 0.5 (t, 0) q.  Double precision, rounded to float at the end.
+extern "C" ftl_status ftl_dualquat_from_matrix(const float* m, float* dq) {
+  if (!m || !dq) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  double R[3][3], t[3];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) R[r][c] = (double)m[4 * r + c];
+    t[r] = (double)m[4 * r + 3];
+  }
+  for (int i = 0; i < 12; ++i)
+    if (!std::isfinite(m[i])) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "non-finite matrix");
+  double q[4];  // x, y, z, w
+  const double tr = R[0][0] + R[1][1] + R[2][2];
+  if (tr > 0.0) {
+    const double s2 = std::sqrt(1.0 + tr), f = 0.5 / s2;
+    q[3] = 0.5 * s2;
+    q[0] = (R[2][1] - R[1][2]) * f;
+    q[1] = (R[0][2] - R[2][0]) * f;
+    q[2] = (R[1][0] - R[0][1]) * f;
+  } else if (R[0][0] > R[1][1] && R[0][0] > R[2][2]) {
+    const double s2 = std::sqrt(1.0 + R[0][0] - R[1][1] - R[2][2]), f = 0.5 / s2;
+    q[0] = 0.5 * s2;
+    q[1] = (R[0][1] + R[1][0]) * f;
+    q[2] = (R[0][2] + R[2][0]) * f;
+    q[3] = (R[2][1] - R[1][2]) * f;
+  } else if (R[1][1] > R[2][2]) {
+    const double s2 = std::sqrt(1.0 + R[1][1] - R[0][0] - R[2][2]), f = 0.5 / s2;
+    q[1] = 0.5 * s2;
+    q[0] = (R[0][1] + R[1][0]) * f;
+    q[2] = (R[1][2] + R[2][1]) * f;
+    q[3] = (R[0][2] - R[2][0]) * f;
+  } else {
+    const double s2 = std::sqrt(1.0 + R[2][2] - R[0][0] - R[1][1]), f = 0.5 / s2;
+    q[2] = 0.5 * s2;
+    q[0] = (R[0][2] + R[2][0]) * f;
+    q[1] = (R[1][2] + R[2][1]) * f;
+    q[3] = (R[1][0] - R[0][1]) * f;
+  }
+  // dual = 0.5 (t, 0) * q: w = -0.5 t.q_xyz, xyz = 0.5 (t q_w + t x q_xyz)
+  const double d[4] = {0.5 * (t[0] * q[3] + t[1] * q[2] - t[2] * q[1]),
+                       0.5 * (t[1] * q[3] + t[2] * q[0] - t[0] * q[2]),
+                       0.5 * (t[2] * q[3] + t[0] * q[1] - t[1] * q[0]),
+                       -0.5 * (t[0] * q[0] + t[1] * q[1] + t[2] * q[2])};
+  for (int c = 0; c < 4; ++c) {
+    dq[c] = (float)q[c];
+    dq[4 + c] = (float)d[c];
+  }
+  return FTL_OK;
+}
+
+extern "C" ftl_status ftl_skin_set_matrices(ftl_skin* s, const float* m3x4, int nbones) {
+  if (!s || !m3x4) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (nbones != s->nbones) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "expected %d bones, got %d", s->nbones, nbones);
+  for (long i = 0; i < 12L * nbones; ++i)
+    if (!std::isfinite(m3x4[i])) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "non-finite bone matrix");
+  std::vector<float> dq((size_t)nbones * 8);
+  for (int b = 0; b < nbones; ++b) {
+    const ftl_status st = ftl_dualquat_from_matrix(m3x4 + 12 * (size_t)b, dq.data() + 8 * (size_t)b);
+    if (st != FTL_OK) return st;
+  }
+  std::memcpy(s->h_mats.data(), m3x4, (size_t)nbones * 48);
+  std::memcpy(s->h_dq.data(), dq.data(), (size_t)nbones * 32);
+  s->dirty = s->mats_dirty = true;
+  return FTL_OK;
+}
+
+extern "C" ftl_status ftl_skin_set_joint_matrices(ftl_skin* s, const float* global_pose, const float* inverse_bind,
+                                                  int nbones) {
+  if (!s || !global_pose) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (nbones != s->nbones) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "expected %d bones, got %d", s->nbones, nbones);
+  std::vector<float> rows((size_t)nbones * 12);
+  for (int b = 0; b < nbones; ++b) {
+    const float* G = global_pose + 16 * (size_t)b;  // column-major: G[4 * col + row]
+    const float* I = inverse_bind ? inverse_bind + 16 * (size_t)b : nullptr;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 4; ++c) {
+        double acc = 0.0;
+        for (int k = 0; k < 4; ++k) {
+          const double ik = I ? (double)I[4 * c + k] : (k == c ? 1.0 : 0.0);
+          acc += (double)G[4 * k + r] * ik;
+        }
+        rows[12 * (size_t)b + 4 * r + c] = (float)acc;
+      }
+  }
+  return ftl_skin_set_matrices(s, rows.data(), nbones);
+}
+
+extern "C" ftl_status ftl_skin_set_rest_normals(ftl_skin* s, const float* normals) {
+  if (!s || !normals) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  for (long i = 0; i < 4L * s->nverts; ++i)
+    if (!std::isfinite(normals[i])) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "non-finite normal");
+  const size_t vb = (size_t)s->nverts * 16;
+  hipError_t e;
+  if ((e = hipSetDevice(s->ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
+  ftl_status st;
+  if (!s->d_nrest && (st = ftl_dev_alloc(s->ctx, &s->d_nrest, vb, "skin rest normals")) != FTL_OK) return st;
+  if (!s->d_nout && (st = ftl_dev_alloc(s->ctx, &s->d_nout, vb, "skin normals")) != FTL_OK) return st;
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return ftl_hip_fail(e, "hipDeviceSynchronize");
+  if ((e = hipMemcpy(s->d_nrest, normals, vb, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(s->d_nout, normals, vb, hipMemcpyHostToDevice)) != hipSuccess)
+    return ftl_hip_fail(e, "hipMemcpy normals");
+  return FTL_OK;
+}
+
 extern "C" ftl_status ftl_skin_update(ftl_skin* s, void* stream) {
   if (!s) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "skin is NULL");
   hipError_t e;
   if ((e = hipSetDevice(s->ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
-  if (s->dirty) {
-    // Synchronous copy: the bones are tiny and the host array is reused.
+  if (s->dirty || s->mats_dirty) {
+    // Synchronous copies: the bones are tiny and the host arrays are reused.
     if ((e = hipStreamSynchronize((hipStream_t)stream)) != hipSuccess) return ftl_hip_fail(e, "hipStreamSynchronize");
-    if ((e = hipMemcpy(s->d_dq, s->h_dq.data(), (size_t)s->nbones * 32, hipMemcpyHostToDevice)) != hipSuccess)
+    if (s->dirty && (e = hipMemcpy(s->d_dq, s->h_dq.data(), (size_t)s->nbones * 32, hipMemcpyHostToDevice)) !=
+                        hipSuccess)
       return ftl_hip_fail(e, "hipMemcpy bones");
-    s->dirty = false;
+    if (s->mats_dirty &&
+        (e = hipMemcpy(s->d_mats, s->h_mats.data(), (size_t)s->nbones * 48, hipMemcpyHostToDevice)) != hipSuccess)
+      return ftl_hip_fail(e, "hipMemcpy bone matrices");
+    s->dirty = s->mats_dirty = false;
   }
-  if ((e = ftl_launch_skin(s->nverts, (const float4*)s->d_dq, (const int4*)s->d_joints, (const float4*)s->d_weights,
-                           (const float4*)s->d_rest, (float4*)s->d_out, (hipStream_t)stream)) != hipSuccess)
+  const float4* bones = (const float4*)(s->mode == FTL_SKIN_LBS ? s->d_mats : s->d_dq);
+  if ((e = ftl_launch_skin(s->mode, s->nverts, bones, (const int4*)s->d_joints, (const float4*)s->d_weights,
+                           (const float4*)s->d_rest, (const float4*)s->d_nrest, (float4*)s->d_out,
+                           (float4*)s->d_nout, (hipStream_t)stream)) != hipSuccess)
     return ftl_hip_fail(e, "skinning launch");
   return FTL_OK;
 }
@@ -699,6 +856,46 @@ extern "C" ftl_status ftl_skin_download(ftl_skin* s, float* verts) {
   if ((e = hipMemcpy(verts, s->d_out, (size_t)s->nverts * 16, hipMemcpyDeviceToHost)) != hipSuccess)
     return ftl_hip_fail(e, "hipMemcpy skinned vertices");
   return FTL_OK;
+}
+
+extern "C" ftl_status ftl_skin_download_normals(ftl_skin* s, float* normals) {
+  if (!s || !normals) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!s->d_nout) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "the skin has no normals (ftl_skin_set_rest_normals)");
+  hipError_t e;
+  if ((e = hipSetDevice(s->ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return ftl_hip_fail(e, "hipDeviceSynchronize");
+  if ((e = hipMemcpy(normals, s->d_nout, (size_t)s->nverts * 16, hipMemcpyDeviceToHost)) != hipSuccess)
+    return ftl_hip_fail(e, "hipMemcpy skinned normals");
+  return FTL_OK;
+}
+
+extern "C" ftl_status ftl_groom_root_frames(ftl_groom* g, float* roots, float* normals) {
+  if (!g || !roots) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!g->skin || !g->d_bind) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "the groom is not bound to a skin");
+  ftl_skin* s = g->skin;
+  if (normals && !s->d_nout)
+    return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "normals asked for, but the skin has none (ftl_skin_set_rest_normals)");
+  const size_t bytes = (size_t)g->nstrands * 16;
+  hipError_t e;
+  if ((e = hipSetDevice(g->ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
+  void *d_r = nullptr, *d_n = nullptr;
+  ftl_status st;
+  if ((st = ftl_dev_alloc(g->ctx, &d_r, bytes, "root frames")) != FTL_OK) return st;
+  if (normals && (st = ftl_dev_alloc(g->ctx, &d_n, bytes, "root normals")) != FTL_OK) {
+    ftl_dev_free(g->ctx, d_r, bytes);
+    return st;
+  }
+  st = FTL_OK;
+  if ((e = hipDeviceSynchronize()) != hipSuccess ||
+      (e = ftl_launch_root_frames(g->nstrands, (const int4*)g->d_bind, (const int4*)s->d_tris,
+                                  (const float4*)s->d_out, normals ? (const float4*)s->d_nout : nullptr,
+                                  (float4*)d_r, (float4*)d_n, nullptr)) != hipSuccess ||
+      (e = hipDeviceSynchronize()) != hipSuccess || (e = hipMemcpy(roots, d_r, bytes, hipMemcpyDeviceToHost)) != hipSuccess ||
+      (normals && (e = hipMemcpy(normals, d_n, bytes, hipMemcpyDeviceToHost)) != hipSuccess))
+    st = ftl_hip_fail(e, "root frames");
+  ftl_dev_free(g->ctx, d_r, bytes);
+  ftl_dev_free(g->ctx, d_n, bytes);
+  return st;
 }
 
 extern "C" ftl_status ftl_groom_bind_skin(ftl_groom* g, ftl_skin* s, const int* tri, const float* bary) {

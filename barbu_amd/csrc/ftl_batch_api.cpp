@@ -153,8 +153,14 @@ extern "C" ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int
   hipError_t e;
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
   const size_t sb = steps.size() * sizeof(FtlStep), tb = tiles.size() * sizeof(FtlTile);
-  // The scratch buffers may still be read by a previous batch step.
-  if ((e = hipStreamSynchronize((hipStream_t)stream)) != hipSuccess) return ftl_hip_fail(e, "hipStreamSynchronize");
+  // The scratch buffers may still be read by the previous batch step, which
+  // may have run on another stream: wait for its kernels before the
+  // descriptors are replaced (or the buffers freed to grow).
+  std::lock_guard<std::mutex> scratch_lock(ctx->scratch_mu);
+  if (!ctx->scratch_ev && (e = hipEventCreateWithFlags(&ctx->scratch_ev, hipEventDisableTiming)) != hipSuccess)
+    return ftl_hip_fail(e, "hipEventCreate");
+  if (ctx->scratch_ev_recorded && (e = hipEventSynchronize(ctx->scratch_ev)) != hipSuccess)
+    return ftl_hip_fail(e, "hipEventSynchronize");
   if (sb > ctx->scratch_steps_bytes) {
     ftl_dev_free(ctx, ctx->scratch_steps, ctx->scratch_steps_bytes);
     ctx->scratch_steps = nullptr;
@@ -176,6 +182,9 @@ extern "C" ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int
   if ((e = ftl_enqueue_runs(runs, (const FtlStep*)ctx->scratch_steps, (const FtlTile*)ctx->scratch_tiles,
                             (hipStream_t)stream, nullptr, nullptr, nullptr)) != hipSuccess)
     return ftl_hip_fail(e, "batched step launch");
+  if ((e = hipEventRecord(ctx->scratch_ev, (hipStream_t)stream)) != hipSuccess)
+    return ftl_hip_fail(e, "hipEventRecord");
+  ctx->scratch_ev_recorded = true;
   return FTL_OK;
 }
 

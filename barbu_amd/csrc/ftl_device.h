@@ -316,8 +316,8 @@ __device__ __forceinline__ void collide(const Spheres& S, float3& p, float3& u, 
 }
 
 // Colliders a strand can touch during one step, as a bit mask.  After every
-// sweep each point lies within the chain length L of its root p0 (segments
-// are sigma*l long); collider j moves a point it contains to its surface, by
+// sweep each point lies within the chain length L = sum |sigma*l| of its root
+// p0 (segments are |sigma*l| long); collider j moves a point it contains to its surface, by
 // at most r_j - |p0 - c_j| + D when D bounds the point's distance to p0, so
 // D grows by that much for each collider that can touch the strand.  A
 // collider farther than r_k + D (with a generous rounding margin) can never

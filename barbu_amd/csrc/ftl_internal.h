@@ -20,7 +20,11 @@ struct ftl_ctx {
   std::mutex mu;
   uint64_t bytes = 0;  // device memory held by the context's objects
   int live_objects = 0;
-  // scratch of ftl_batch_step
+  // scratch of ftl_batch_step, and the event recorded after the last batch
+  // that reads it (on whatever stream that batch ran)
+  std::mutex scratch_mu;
+  hipEvent_t scratch_ev = nullptr;
+  bool scratch_ev_recorded = false;
   void* scratch_steps = nullptr;
   size_t scratch_steps_bytes = 0;
   void* scratch_tiles = nullptr;
@@ -30,14 +34,19 @@ struct ftl_ctx {
 struct ftl_skin {
   ftl_ctx* ctx = nullptr;
   int nverts = 0, ntris = 0, nbones = 0;
+  int mode = FTL_SKIN_DQ;
+  bool has_no_joint = false;  // some joint index is FTL_NO_JOINT (LBS only)
   void* d_rest = nullptr;
   void* d_joints = nullptr;
   void* d_weights = nullptr;
   void* d_out = nullptr;
   void* d_tris = nullptr;
-  void* d_dq = nullptr;
-  std::vector<float> h_dq;
-  bool dirty = false;
+  void* d_dq = nullptr;    // 2 float4 per bone
+  void* d_mats = nullptr;  // 3 float4 per bone (rows of the skinning matrix)
+  void* d_nrest = nullptr; // normals (float4 per vertex), or null
+  void* d_nout = nullptr;
+  std::vector<float> h_dq, h_mats;
+  bool dirty = false, mats_dirty = false;
 };
 
 struct ftl_groom {
@@ -73,6 +82,8 @@ struct ftl_expand {
 };
 
 ftl_status ftl_fail(ftl_status st, const char* fmt, ...);
+// Validates simulation parameters (ranges, finiteness).
+ftl_status ftl_check_params(const ftl_params* p);
 // ftl_groom_create without a host state: zero state when desc->positions is
 // NULL (filled on the device afterwards).
 ftl_status ftl_groom_create_uninitialised(ftl_ctx* ctx, const ftl_groom_desc* d, ftl_groom** out);

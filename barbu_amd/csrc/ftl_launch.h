@@ -11,15 +11,24 @@ int ftl_class_m(int cls);
 int ftl_class_g(int cls);
 int ftl_class_guarded(int cls);
 
-hipError_t ftl_launch_plain(int cls, const FtlStep& A, hipStream_t stream);
+// start / stop: optional events bracketing the kernel itself (timing).
+hipError_t ftl_launch_plain(int cls, const FtlStep& A, hipStream_t stream, hipEvent_t start = nullptr,
+                            hipEvent_t stop = nullptr);
 // Tile-list kernel of one class over tiles[0..ntiles) (all of that class):
 // workgroup b steps tiles b, b + grid, ...
 hipError_t ftl_launch_tiles(int cls, const FtlStep* grooms, const FtlTile* tiles, int ntiles, int grid,
                             hipStream_t stream);
 // Resident tile-list workgroups per CU for a class.
 int ftl_tiles_blocks_per_cu(int cls);
-hipError_t ftl_launch_skin(int nverts, const float4* dq, const int4* joints, const float4* weights,
-                           const float4* rest, float4* out, hipStream_t stream);
+// Skinning of mesh vertices (and normals when nout is not null): mode 0 dual
+// quaternions (2 float4 per bone), mode 1 linear blending (3 float4 rows per
+// bone, joint 0xff = identity).
+hipError_t ftl_launch_skin(int mode, int nverts, const float4* bones, const int4* joints, const float4* weights,
+                           const float4* rest, const float4* nrest, float4* out, float4* nout, hipStream_t stream);
+// Per strand of a skinned groom: the root on its bound triangle and, when
+// normals is not null, the unit normal there.
+hipError_t ftl_launch_root_frames(int nstrands, const int4* bind, const int4* tris, const float4* verts,
+                                  const float4* normals, float4* roots, float4* nout, hipStream_t stream);
 // Copies bytes (a multiple of 16) from pinned, coherent host memory to the
 // device with a kernel on `stream`.
 hipError_t ftl_launch_upload(const void* host_src, void* dst, size_t bytes, hipStream_t stream);

@@ -2,6 +2,8 @@
 // time with -DFTL_CLASS_ID=<id>: one wave64 workgroup per tile of strands, or,
 // for launches with wind, workgroups of four tiles sharing an LDS copy of the
 // wind lattice; and the tile-list kernel of batched frames.
+#include <hip/hip_ext.h>
+
 #include "ftl_classes.h"
 #include "ftl_launch.h"
 #include "ftl_tile.h"
@@ -105,16 +107,19 @@ void ftl_plain_wind_kernel(FtlStep arg) {
 #define FTL_PASTE2(a, b) a##b
 #define FTL_PASTE(a, b) FTL_PASTE2(a, b)
 
-hipError_t FTL_PASTE(ftl_launch_plain_, FTL_CLASS_ID)(const FtlStep& A, hipStream_t stream) {
+// start / stop (may be null): events that take the kernel's own start and end
+// timestamps (hipExtLaunchKernelGGL), for per-launch timing.
+hipError_t FTL_PASTE(ftl_launch_plain_, FTL_CLASS_ID)(const FtlStep& A, hipStream_t stream, hipEvent_t start,
+                                                      hipEvent_t stop) {
   constexpr int spt = FTL_WAVE / CL::G;
   const int nblocks = (A.nstrands + spt - 1) / spt;
   if (nblocks <= 0) return hipSuccess;
   if (kWindLds && FTL_WIND_LDS && A.wind_grid) {
-    hipLaunchKernelGGL(ftl_plain_wind_kernel, dim3((nblocks + kWindWaves - 1) / kWindWaves), dim3(FTL_WAVE * kWindWaves),
-                       0, stream, A);
+    hipExtLaunchKernelGGL(ftl_plain_wind_kernel, dim3((nblocks + kWindWaves - 1) / kWindWaves),
+                          dim3(FTL_WAVE * kWindWaves), 0, stream, start, stop, 0, A);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(ftl_plain_kernel, dim3(nblocks), dim3(FTL_WAVE), 0, stream, A);
+  hipExtLaunchKernelGGL(ftl_plain_kernel, dim3(nblocks), dim3(FTL_WAVE), 0, stream, start, stop, 0, A);
   return hipGetLastError();
 }
 

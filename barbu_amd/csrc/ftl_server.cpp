@@ -26,6 +26,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
@@ -55,6 +56,10 @@ struct Creature {
   ftl_params params;
   uint64_t bytes = 0;  // device bytes it holds (or is expected to hold)
   bool inflight = false;  // a batch (or a host copy) uses it
+  // A step left non-finite values at a strand tip: the creature is left out
+  // of batches (its requests complete as FTL_ERR_NONFINITE) until its state
+  // is replaced with ftl_server_upload.
+  bool quarantined = false;
   // host copies kept while waiting for memory
   std::vector<float> pos, vel, roots;
   bool has_roots = false;
@@ -71,6 +76,12 @@ struct Slot {
   FtlTile* h_tiles = nullptr;
   FtlTile* d_tiles = nullptr;
   size_t tiles_cap = 0;
+  // Non-finite flag of each request of the batch: pinned, coherent host
+  // memory the step kernels write directly (the groom's status word is
+  // replaced by flags[i] in the batch's descriptors).
+  unsigned* h_flags = nullptr;
+  unsigned* d_flags = nullptr;
+  size_t flags_cap = 0;
   std::vector<Req> reqs;
   bool busy = false;
 };
@@ -85,6 +96,10 @@ struct ftl_server {
   ftl_ctx* ctx = nullptr;
   uint64_t budget = 0, batch_cap = 0;
   int nslots = 2;
+  // coalescing: a batch waits for this much work, or for its oldest request
+  // to be coalesce_us old, while another batch is on the device
+  uint64_t batch_target = 0;
+  int coalesce_us = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::map<int, Creature> creatures;
@@ -93,6 +108,9 @@ struct ftl_server {
   uint64_t resident_bytes = 0;
   std::list<Req> pending;
   std::unordered_set<uint64_t> open;
+  // tickets that completed without a step (FTL_ERR_NONFINITE: quarantined
+  // creature), until ftl_server_wait reports them
+  std::unordered_map<uint64_t, ftl_status> failed;
   uint64_t next_ticket = 1;
   uint64_t work_gen = 0;  // bumped whenever something may have become launchable
   Slot slots[kMaxSlots];
@@ -103,6 +121,7 @@ struct ftl_server {
   // statistics
   std::vector<float> lat_us;
   uint64_t submitted = 0, completed = 0, batches = 0, cps = 0, batch_reqs = 0;
+  uint64_t cps_finite = 0, dropped = 0, failed_admissions = 0;
   int busy_slots = 0;
   Clock::time_point busy_since, first_submit, last_complete, stats_since;
   double busy_total_s = 0.0;
@@ -137,6 +156,24 @@ ftl_status make_resident(ftl_server* s, Creature& c) {
   return FTL_OK;
 }
 
+// Removes the pending requests of creature id; their tickets complete with
+// status `why` (FTL_OK: cancelled by a removal) (lock held).
+void drop_requests(ftl_server* s, int id, ftl_status why = FTL_OK) {
+  for (auto it = s->pending.begin(); it != s->pending.end();) {
+    if (it->id == id) {
+      s->open.erase(it->ticket);
+      if (why != FTL_OK) {
+        s->failed[it->ticket] = why;
+        s->dropped++;
+      }
+      it = s->pending.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  s->cv_done.notify_all();
+}
+
 // Admits waiting creatures, in arrival order, while they fit (lock held).
 void admit_waiting(ftl_server* s) {
   while (!s->waiting.empty()) {
@@ -146,8 +183,17 @@ void admit_waiting(ftl_server* s) {
       continue;
     }
     if (s->resident_bytes + it->second.bytes > s->budget) break;
-    if (make_resident(s, it->second) != FTL_OK) break;  // device memory short: retry at the next removal
+    const ftl_status st = make_resident(s, it->second);
+    if (st == FTL_ERR_OUT_OF_MEMORY) break;  // device memory short: retry at the next removal
     s->waiting.pop_front();
+    if (st != FTL_OK) {
+      // Any other failure is the creature's own: it is dropped (its queued
+      // requests complete as cancelled) and the creatures behind it go on.
+      const int id = it->first;
+      drop_requests(s, id);
+      s->creatures.erase(it);
+      s->failed_admissions++;
+    }
   }
   s->work_gen++;
   s->cv_work.notify_all();
@@ -186,6 +232,17 @@ hipError_t ensure_slot(ftl_server* s, Slot& sl, size_t nsteps, size_t ntiles, si
     if (ftl_dev_alloc(s->ctx, (void**)&sl.d_tiles, cap * sizeof(FtlTile), "server tiles") != FTL_OK)
       return hipErrorOutOfMemory;
     sl.tiles_cap = cap;
+  }
+  if (nsteps > sl.flags_cap) {
+    const size_t cap = std::max(nsteps, sl.flags_cap * 2);
+    if (sl.h_flags) (void)hipHostFree(sl.h_flags);
+    sl.h_flags = nullptr;
+    sl.d_flags = nullptr;
+    sl.flags_cap = 0;
+    if ((e = hipHostMalloc((void**)&sl.h_flags, cap * sizeof(unsigned), hipHostMallocCoherent)) != hipSuccess)
+      return e;
+    if ((e = hipHostGetDevicePointer((void**)&sl.d_flags, sl.h_flags, 0)) != hipSuccess) return e;
+    sl.flags_cap = cap;
   }
   while (sl.side.size() + 1 < nruns) {
     hipStream_t st = nullptr;
@@ -240,8 +297,11 @@ ftl_status launch_batch(ftl_server* s, Slot& sl) {
   if (st != FTL_OK) return st;
   hipError_t e = ensure_slot(s, sl, n, tiles.size(), runs.size());
   if (e != hipSuccess) return ftl_hip_fail(e, "server slot buffers");
-  for (size_t i = 0; i < n; ++i)
+  for (size_t i = 0; i < n; ++i) {
     if ((st = ftl_make_step(grooms[i], dts[i], &sl.h_steps[i])) != FTL_OK) return st;
+    sl.h_flags[i] = 0u;
+    sl.h_steps[i].status = sl.d_flags + i;
+  }
   std::memcpy(sl.h_tiles, tiles.data(), tiles.size() * sizeof(FtlTile));
   if (FTL_KERNEL_UPLOAD) {
     if ((e = ftl_launch_upload(sl.h_steps, sl.d_steps, n * sizeof(FtlStep), sl.s)) != hipSuccess)
@@ -262,43 +322,78 @@ ftl_status launch_batch(ftl_server* s, Slot& sl) {
 }
 
 // Requests of one batch: in arrival order, one per creature, idle resident
-// creatures only, up to the traffic cap (lock held).
-void select_batch(ftl_server* s, Slot& sl) {
+// creatures only, up to the traffic cap (lock held).  Requests of a
+// quarantined creature complete at once with FTL_ERR_NONFINITE.  With
+// coalescing, while another batch is on the device, the selection is taken
+// only once it holds batch_target bytes or its oldest request is coalesce_us
+// old; otherwise nothing is taken and *deadline says when to look again.
+void select_batch(ftl_server* s, Slot& sl, Clock::time_point now, Clock::time_point* deadline) {
   sl.reqs.clear();
   uint64_t bytes = 0;
   std::unordered_set<int> blocked;
+  std::vector<std::list<Req>::iterator> take;
+  Clock::time_point oldest = now;
   for (auto it = s->pending.begin(); it != s->pending.end();) {
     const int id = it->id;
     Creature& c = s->creatures.at(id);
+    if (c.quarantined && !c.inflight && !blocked.count(id)) {
+      s->open.erase(it->ticket);
+      s->failed[it->ticket] = FTL_ERR_NONFINITE;
+      s->dropped++;
+      it = s->pending.erase(it);
+      s->cv_done.notify_all();
+      continue;
+    }
     if (blocked.count(id) || !c.g || c.inflight) {
       blocked.insert(id);  // keeps the creature's later requests behind this one
       ++it;
       continue;
     }
     const uint64_t b = 64ull * (uint64_t)c.nstrands * (uint64_t)c.npoints;
-    if (!sl.reqs.empty() && bytes + b > s->batch_cap) break;
+    if (!take.empty() && bytes + b > s->batch_cap) break;
     bytes += b;
-    c.inflight = true;
     blocked.insert(id);
+    oldest = std::min(oldest, it->t0);
+    take.push_back(it);
+    ++it;
+  }
+  if (take.empty()) return;
+  if (s->coalesce_us > 0 && s->busy_slots > 0 && bytes < s->batch_target) {
+    const Clock::time_point due = oldest + std::chrono::microseconds(s->coalesce_us);
+    if (now < due) {
+      *deadline = std::min(*deadline, due);
+      return;
+    }
+  }
+  for (auto it : take) {
+    Creature& c = s->creatures.at(it->id);
+    c.inflight = true;
     it->cps = (uint64_t)c.nstrands * (uint64_t)c.npoints;
     sl.reqs.push_back(*it);
-    it = s->pending.erase(it);
+    s->pending.erase(it);
   }
 }
 
 // Hands a finished batch's requests back.  Statistics count the requests
-// submitted since the last reset.
-void complete_slot(ftl_server* s, Slot& sl, Clock::time_point now) {
-  for (const Req& q : sl.reqs) {
+// submitted since the last reset.  A request whose step flagged non-finite
+// values quarantines its creature.
+void complete_slot(ftl_server* s, Slot& sl, Clock::time_point now, bool ok = true) {
+  for (size_t i = 0; i < sl.reqs.size(); ++i) {
+    const Req& q = sl.reqs[i];
+    const bool bad = ok && sl.h_flags && __atomic_load_n(&sl.h_flags[i], __ATOMIC_ACQUIRE) != 0u;
     if (q.t0 >= s->stats_since) {
       s->lat_us.push_back((float)us_between(q.t0, now));
       s->completed++;
       s->cps += q.cps;
+      if (!bad) s->cps_finite += q.cps;
       s->last_complete = now;
     }
     s->open.erase(q.ticket);
     auto it = s->creatures.find(q.id);
-    if (it != s->creatures.end()) it->second.inflight = false;
+    if (it != s->creatures.end()) {
+      it->second.inflight = false;
+      if (bad) it->second.quarantined = true;
+    }
   }
   sl.reqs.clear();
   sl.busy = false;
@@ -322,14 +417,15 @@ void dispatcher(ftl_server* s) {
         progressed = true;
       } else if (e != hipErrorNotReady) {
         set_fatal(s, ftl_hip_fail(e, "server batch"), "batch failed on the device");
-        complete_slot(s, sl, Clock::now());
+        complete_slot(s, sl, Clock::now(), false);
       }
     }
+    Clock::time_point deadline = Clock::time_point::max();
     if (s->fatal == FTL_OK && !s->pending.empty()) {
       for (int k = 0; k < s->nslots; ++k) {
         Slot& sl = s->slots[k];
         if (sl.busy) continue;
-        select_batch(s, sl);
+        select_batch(s, sl, Clock::now(), &deadline);
         if (sl.reqs.empty()) break;
         sl.busy = true;
         if (s->busy_slots++ == 0) s->busy_since = Clock::now();
@@ -340,19 +436,24 @@ void dispatcher(ftl_server* s) {
         lk.lock();
         if (st != FTL_OK) {
           set_fatal(s, st, ftl_last_error());
-          complete_slot(s, sl, Clock::now());
+          complete_slot(s, sl, Clock::now(), false);
         }
         progressed = true;
       }
     }
     if (!progressed) {
-      // Batches in flight are polled every 20 us; otherwise sleep until a
-      // submission, admission or release may have made work launchable.
+      // Batches in flight are polled every 20 us (sooner when a coalescing
+      // window closes); otherwise sleep until a submission, admission or
+      // release may have made work launchable.
       auto wake = [&] { return s->stop || s->work_gen != gen; };
-      if (s->busy_slots > 0)
-        s->cv_work.wait_for(lk, std::chrono::microseconds(20), wake);
-      else
+      if (s->busy_slots > 0) {
+        Clock::time_point until = Clock::now() + std::chrono::microseconds(20);
+        s->cv_work.wait_until(lk, std::min(until, deadline), wake);
+      } else if (deadline != Clock::time_point::max()) {
+        s->cv_work.wait_until(lk, deadline, wake);
+      } else {
         s->cv_work.wait(lk, wake);
+      }
     }
   }
 }
@@ -396,8 +497,12 @@ extern "C" ftl_status ftl_server_create(ftl_ctx* ctx, const ftl_server_config* c
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
   ftl_server* s = new ftl_server();
   s->ctx = ctx;
+  if (c.coalesce_us < 0 || c.coalesce_us > 1000000)
+    return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "coalesce_us must be in [0, 1000000] (got %d)", c.coalesce_us);
   s->nslots = c.max_inflight ? c.max_inflight : 2;
   s->batch_cap = c.batch_cap_bytes ? c.batch_cap_bytes : kDefaultBatchCap;
+  s->coalesce_us = c.coalesce_us;
+  s->batch_target = c.batch_target_bytes ? std::min(c.batch_target_bytes, s->batch_cap) : s->batch_cap;
   s->budget = c.hbm_budget_bytes;
   if (!s->budget) {
     size_t free_b = 0, total_b = 0;
@@ -455,6 +560,7 @@ extern "C" void ftl_server_destroy(ftl_server* s) {
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.h_steps) (void)hipHostFree(sl.h_steps);
     if (sl.h_tiles) (void)hipHostFree(sl.h_tiles);
+    if (sl.h_flags) (void)hipHostFree(sl.h_flags);
     ftl_dev_free(s->ctx, sl.d_steps, sl.steps_cap * sizeof(FtlStep));
     ftl_dev_free(s->ctx, sl.d_tiles, sl.tiles_cap * sizeof(FtlTile));
   }
@@ -471,6 +577,13 @@ extern "C" ftl_status ftl_server_add(ftl_server* s, const ftl_groom_desc* d, con
   if (d->nstrands <= 0 || d->npoints < 2 || d->npoints > FTL_MAX_POINTS || !d->positions)
     return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "bad groom description (nstrands %d, npoints %d)", d->nstrands,
                     d->npoints);
+  // Validated now, whether or not the creature becomes resident at once: a
+  // queued creature with bad parameters would otherwise fail only when
+  // admitted, and block every creature behind it.
+  {
+    const ftl_status pst = ftl_check_params(p);
+    if (pst != FTL_OK) return pst;
+  }
   Creature c;
   c.nstrands = d->nstrands;
   c.npoints = d->npoints;
@@ -512,14 +625,7 @@ extern "C" ftl_status ftl_server_remove(ftl_server* s, int id) {
   if (!c) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "unknown creature %d", id);
   s->cv_done.wait(lk, [&] { c = find(s, id); return !c || !c->inflight; });
   if (!c) return FTL_OK;
-  for (auto it = s->pending.begin(); it != s->pending.end();) {
-    if (it->id == id) {
-      s->open.erase(it->ticket);
-      it = s->pending.erase(it);
-    } else {
-      ++it;
-    }
-  }
+  drop_requests(s, id);
   if (c->g) {
     ftl_groom_destroy(c->g);
     s->resident_bytes -= c->bytes;
@@ -596,6 +702,13 @@ extern "C" ftl_status ftl_server_wait(ftl_server* s, uint64_t ticket, int timeou
   else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done))
     return ftl_fail(FTL_ERR_BUSY, "request %llu still running", (unsigned long long)ticket);
   if (s->fatal != FTL_OK) return ftl_fail(s->fatal, "server stopped: %s", s->fatal_msg.c_str());
+  auto f = s->failed.find(ticket);
+  if (f != s->failed.end()) {
+    const ftl_status st = f->second;
+    s->failed.erase(f);
+    return ftl_fail(st, "request %llu was not stepped: its creature is quarantined (non-finite state)",
+                    (unsigned long long)ticket);
+  }
   return FTL_OK;
 }
 
@@ -641,10 +754,23 @@ extern "C" ftl_status ftl_server_status(ftl_server* s, int id, int* nonfinite) {
   Creature* c = nullptr;
   ftl_status st = reserve(s, lk, id, &c);
   if (st != FTL_OK) return st;
+  *nonfinite = c->quarantined ? 1 : 0;
+  release(s, id);
+  return FTL_OK;
+}
+
+extern "C" ftl_status ftl_server_upload(ftl_server* s, int id, const float* pos, const float* vel) {
+  if (!s || !pos || !vel) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  Creature* c = nullptr;
+  ftl_status st = reserve(s, lk, id, &c);
+  if (st != FTL_OK) return st;
   ftl_groom* g = c->g;
   lk.unlock();
-  st = ftl_groom_status(g, nonfinite);
+  st = ftl_groom_upload(g, pos, vel);
   lk.lock();
+  c = find(s, id);
+  if (c && st == FTL_OK) c->quarantined = false;
   release(s, id);
   return st;
 }
@@ -682,6 +808,12 @@ extern "C" ftl_status ftl_server_get_stats(ftl_server* s, ftl_server_stats* out)
   for (const auto& kv : s->creatures) res += kv.second.g ? 1 : 0;
   r.resident = res;
   r.waiting = (int)s->waiting.size();
+  r.cps_finite = s->cps_finite;
+  r.dropped = s->dropped;
+  r.failed_admissions = s->failed_admissions;
+  int q = 0;
+  for (const auto& kv : s->creatures) q += kv.second.quarantined ? 1 : 0;
+  r.quarantined = q;
   *out = r;
   return FTL_OK;
 }
@@ -691,6 +823,7 @@ extern "C" ftl_status ftl_server_reset_stats(ftl_server* s) {
   std::lock_guard<std::mutex> lk(s->mu);
   s->lat_us.clear();
   s->submitted = s->completed = s->batches = s->cps = s->batch_reqs = 0;
+  s->cps_finite = s->dropped = 0;
   s->busy_total_s = 0.0;
   if (s->busy_slots > 0) s->busy_since = Clock::now();
   s->any_submit = false;

@@ -515,7 +515,9 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     float L = 0.0f;
 #pragma unroll
     for (int j = 0; j < M; ++j)
-      if (j >= (G == 1 ? 1 : 0) && (!GUARD || j < nl)) L += A.sigma * l.get(j);
+      // |sigma * l|: a negative length scale or rest length still places a
+      // point that far from its leader.
+      if (j >= (G == 1 ? 1 : 0) && (!GUARD || j < nl)) L += fabsf(A.sigma * l.get(j));
     float3 root = p[0];
     if (G > 1) {
       // strand total and root, from the lanes of the strand
@@ -616,7 +618,11 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
   }
   bad = bad && valid;
   const unsigned long long b = __ballot(bad);
-  if (b != 0ull && lane == (int)(__ffsll((long long)b) - 1)) atomicOr(A.status, 1u);
+  // A plain (system-scope) store of 1, not a read-modify-write: the word may
+  // live in pinned host memory (the serving loop's per-request flags), and
+  // every writer stores the same value.
+  if (b != 0ull && lane == (int)(__ffsll((long long)b) - 1))
+    __hip_atomic_store(A.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 }  // namespace ftl_tile_impl

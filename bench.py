@@ -8,12 +8,18 @@ wind, rotating emitter, two colliders, dt = 1/60), inputs resident in HBM.
 The other workloads are reported in extra keys of the same line.
 
 Timing: W untimed steps, then exactly K steps bracketed by device
-synchronisation; HIP events recorded on the stream the kernels run on give
-the per-launch duration used by the roofline figure.
+synchronisation.  `roofline.kernel_us` is the mean duration of the step
+kernel itself: each timed launch is bracketed by two HIP events that take the
+kernel's own start and end timestamps (ftl_groom_step_timed), on the stream the
+kernel runs on.  The window of the whole timed region on that stream is
+reported beside it (`stream_window_us`).
 """
 import argparse
+import ctypes as C
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -22,38 +28,23 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from barbu_amd import workloads as wl  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 BYTES_PER_CP = 64      # read + write of float4 position and float4 velocity
-DT = np.float32(1.0 / 60.0)
+DT = wl.DT
 METRIC = "FTL control-point updates/s & % HBM peak; p99 step latency, batched vs plain"
+TUNED = os.path.join(ROOT, "barbu_amd", "tuned_gfx950.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def c2_frame(t):
-    """Per-step inputs of the single-groom workload at time t: emitter rotation
-    about y at 1 rad/s, sphere r=1 at the origin and r=0.3 orbiting the y axis
-    at radius 1.2, 1 rad/s."""
-    c, s = np.cos(t), np.sin(t)
-    m = np.array([[c, 0, s, 0], [0, 1, 0, 0], [-s, 0, c, 0], [0, 0, 0, 1]], np.float32)
-    sph = np.array([[0, 0, 0, 1.0], [1.2 * np.cos(t), 0.0, 1.2 * np.sin(t), 0.3]], np.float32)
-    return m, sph
-
-
-def c2_params(ba):
-    return ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0, iterations=8,
-                          wind_amplitude=2.0, wind_frequency=2.0, wind_scroll=(0.5, 0.0, 0.25), wind_seed=2)
-
-
-TUNED = os.path.join(ROOT, "barbu_amd", "tuned_gfx950.json")
-
-
 def context(ba):
-    """A context with the lanes-per-strand table measured on MI355X."""
+    """A context with the launch-shape tables measured on MI355X."""
     ctx = ba.Context(0)
-    ctx.autotune(TUNED)  # loads the table (no timing) when it matches the device
+    ctx.autotune(TUNED)  # loads the tables (no timing) when they match the device
     return ctx
 
 
@@ -61,45 +52,59 @@ def bench_single(ba, hipm, steps, warmup, S=262144, N=32, lanes=None):
     pos, vel, roots = ba.sphere_groom(S, N, radius=1.0, length=0.2, jitter=0.05, seed=2024)
     ctx = context(ba)
     g = ba.Groom(ctx, pos, vel, rest_roots=roots)
-    g.set_params(c2_params(ba))
+    g.set_params(wl.c2_params(ba))
     g.set_root_mode(ba.FTL_ROOT_REST)
     if lanes:
         g.set_lanes(lanes)
+    del pos, vel, roots
     stream = hipm.Stream()
     e0, e1 = hipm.Event(), hipm.Event()
+    # per-step inputs, evaluated before the timed region
+    total = warmup + steps
+    times = [float(k + 1) * float(DT) for k in range(total)]
+    cms, sphs = [], []
+    for t in times:
+        m, sph = wl.c2_frame(t)
+        cms.append(np.ascontiguousarray(m.T))
+        sphs.append(np.ascontiguousarray(sph))
+    ks = [(hipm.Event(), hipm.Event()) for _ in range(steps)]
+    L = ba.lib()
+    h = g.handle
+    dt = float(DT)
 
-    def step(k):
-        t = float(k + 1) * float(DT)
-        m, sph = c2_frame(t)
-        g.set_root_transform(m)
-        g.set_colliders(sph)
-        g.set_time(t)
-        g.step(DT, stream.h)
+    def inputs(k):
+        ba.check(L.ftl_groom_set_root_transform(h, cms[k].ctypes.data), "ftl_groom_set_root_transform")
+        ba.check(L.ftl_groom_set_colliders(h, sphs[k].ctypes.data, 2), "ftl_groom_set_colliders")
+        ba.check(L.ftl_groom_set_time(h, times[k]), "ftl_groom_set_time")
 
     for k in range(warmup):
-        step(k)
+        inputs(k)
+        ba.check(L.ftl_groom_step(h, dt, stream.h), "ftl_groom_step")
     hipm.device_synchronize()
     w0 = time.perf_counter()
     e0.record(stream.h)
-    for k in range(warmup, warmup + steps):
-        step(k)
+    for i, k in enumerate(range(warmup, total)):
+        inputs(k)
+        ba.check(L.ftl_groom_step_timed(h, dt, stream.h, ks[i][0].h, ks[i][1].h), "ftl_groom_step_timed")
     e1.record(stream.h)
     hipm.device_synchronize()
     wall = time.perf_counter() - w0
     gpu_ms = e0.elapsed_ms(e1)
+    kern_ms = np.array([a.elapsed_ms(b) for a, b in ks])
     p, v = g.download()
     assert np.isfinite(p).all() and np.isfinite(v).all(), "non-finite state after the benchmark"
     lanes = lanes or ctx.lanes_for(N, wind=True)
     g.close()
     copy_gbs = copy_reference(ba, hipm, ctx, S * N * 32)
     ctx.close()
-    return dict(S=S, N=N, wall_s=wall, gpu_ms=gpu_ms, lanes=lanes, copy_gbs=copy_gbs)
+    return dict(S=S, N=N, wall_s=wall, gpu_ms=gpu_ms, lanes=lanes, copy_gbs=copy_gbs,
+                kernel_us_mean=float(kern_ms.mean() * 1000.0), kernel_us_median=float(np.median(kern_ms) * 1000.0),
+                kernel_us_min=float(kern_ms.min() * 1000.0), kernel_us_max=float(kern_ms.max() * 1000.0))
 
 
 def copy_reference(ba, hipm, ctx, nbytes, reps=20):
     """GB/s of a plain streaming copy of nbytes (the step's read + write
     volume is 2 x nbytes): the practical bandwidth ceiling on this device."""
-    import ctypes as C
     L = ba.lib()
     a, b = C.c_void_p(), C.c_void_p()
     ba.check(L.ftl_diag_alloc(ctx.handle, nbytes, C.byref(a)), "ftl_diag_alloc")
@@ -119,45 +124,71 @@ def copy_reference(ba, hipm, ctx, nbytes, reps=20):
     return 2.0 * nbytes / (ms / 1000.0) / 1e9
 
 
+def host_info():
+    """The host the CPU baseline ran on: CPU model, nproc, the cores this
+    process may run on, and the thread count the environment grants."""
+    model = platform.processor() or ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc", "--all"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        nproc = os.cpu_count() or 1
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return dict(cpu_model=model, nproc=nproc, affinity_cores=affinity, omp_num_threads=omp or None)
+
+
+def cpu_threads(info):
+    """Every core the process may run on, within the thread count the
+    environment grants (OMP_NUM_THREADS: a GPU box's host share)."""
+    n = info["affinity_cores"]
+    if info["omp_num_threads"]:
+        try:
+            n = min(n, int(info["omp_num_threads"]))
+        except ValueError:
+            pass
+    return max(1, n)
+
+
 def cpu_baseline(ba, seconds_target=12.0):
     """The FP32 oracle (a plain C++ transliteration of the reference shader)
     on the host cores, on a bounded sample of the single-groom workload."""
     from oracle import ftl_oracle as orc
     from tests.helpers import oracle_params_for
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    info = host_info()
+    threads = cpu_threads(info)
     S, N = 32768, 32  # 1/8 of the strands of the workload
     pos, vel, roots = ba.sphere_groom(S, N, radius=1.0, length=0.2, jitter=0.05, seed=2024)
-    params = c2_params(ba)
+    params = wl.c2_params(ba)
     wind = ba.wind_table(params)
-    done, k, t0 = 0, 0, time.perf_counter()
-    while True:
-        t = float(k + 1) * float(DT)
-        m, sph = c2_frame(t)
-        op = oracle_params_for(params, N, DT, spheres=[tuple(x) for x in sph], model=m, root_mode=1, time=t,
-                               wind=wind)
-        orc.step_f32(op, pos, vel, roots, nthreads=threads)
-        done += S * N
-        k += 1
-        el = time.perf_counter() - t0
-        if el >= seconds_target or k >= 2000:
-            break
-    # single thread, same sample, about 2 s
-    done1, k1, t1 = 0, 0, time.perf_counter()
-    while True:
-        t = float(k + k1 + 1) * float(DT)
-        m, sph = c2_frame(t)
-        op = oracle_params_for(params, N, DT, spheres=[tuple(x) for x in sph], model=m, root_mode=1, time=t,
-                               wind=wind)
-        orc.step_f32(op, pos, vel, roots, nthreads=1)
-        done1 += S * N
-        k1 += 1
-        el1 = time.perf_counter() - t1
-        if el1 >= seconds_target / 6.0:
-            break
+
+    def run(nthreads, seconds, k0):
+        done, k, t0 = 0, 0, time.perf_counter()
+        while True:
+            t = float(k0 + k + 1) * float(DT)
+            m, sph = wl.c2_frame(t)
+            op = oracle_params_for(params, N, DT, spheres=[tuple(x) for x in sph], model=m, root_mode=1, time=t,
+                                   wind=wind)
+            orc.step_f32(op, pos, vel, roots, nthreads=nthreads)
+            done += S * N
+            k += 1
+            el = time.perf_counter() - t0
+            if el >= seconds or k >= 2000:
+                return done / el, k, el
+
+    v_all, k_all, el_all = run(threads, seconds_target, 0)
+    v_one, _, _ = run(1, seconds_target / 6.0, k_all)
     # the CPU configuration: 4,096 x 16, one sphere, 100 steps
     c1 = {}
-    for nt in (1, threads):
+    for nt in sorted({1, threads}):
         p1, v1, _ = ba.sphere_groom(4096, 16, radius=1.0, length=0.2, jitter=0.05, seed=1, coherent=False)
         op = oracle_params_for(ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0), 16, DT,
                                spheres=[(0.0, 0.0, 0.0, 1.0)])
@@ -166,114 +197,16 @@ def cpu_baseline(ba, seconds_target=12.0):
             orc.step_f32(op, p1, v1, nthreads=nt)
         c1["threads_%d_s" % nt] = time.perf_counter() - c0
         c1["threads_%d_cps" % nt] = 4096 * 16 * 100 / c1["threads_%d_s" % nt]
-    return dict(value=done / el, unit="CP updates/s", cores=threads, kind="port",
+    return dict(value=v_all, unit="CP updates/s", cores=threads, kind="port",
                 sample="%d strands x %d points (1/8 of the single-groom workload), %d steps, %.1f s, "
-                       "OpenMP over strands" % (S, N, k, el),
-                single_thread_value=done1 / el1, c1_4096x16_100_steps=c1)
-
-
-def c3_grooms(seed=3, ngrooms=64):
-    """Composition of the batched workload: 64 grooms, strand counts
-    log-uniform in [1024, 262144], points per strand from {8,16,32,64},
-    lengths U(0.05, 0.4), a random rigid motion each (rotation about a random
-    axis at 0.5-2 rad/s plus a swaying translation), 1-3 sphere colliders that
-    move with the groom (a head sphere r=1 at its centre, 0-2 more)."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(ngrooms):
-        S = int(round(np.exp(rng.uniform(np.log(1024), np.log(262144)))))
-        N = int(rng.choice([8, 16, 32, 64]))
-        L = float(rng.uniform(0.05, 0.4))
-        axis = rng.normal(size=3)
-        axis /= np.linalg.norm(axis)
-        w = float(rng.uniform(0.5, 2.0))
-        sway = rng.uniform(0.0, 0.2, size=3)
-        ncol = int(rng.integers(1, 4))
-        extra = []
-        for _ in range(ncol - 1):
-            d = rng.normal(size=3)
-            d /= np.linalg.norm(d)
-            extra.append((*(d * rng.uniform(1.0, 1.5)), rng.uniform(0.2, 0.5)))
-        center = np.array([(i % 8) * 3.0, 0.0, (i // 8) * 3.0])
-        out.append(dict(S=S, N=N, L=L, axis=axis, w=w, sway=sway, spheres=[(0, 0, 0, 1.0)] + extra,
-                        center=center, seed=1000 + i))
-    return out
-
-
-def rigid(c, t):
-    """Row-major 4x4 world transform of groom c at time t."""
-    a = c["w"] * t
-    k = c["axis"]
-    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
-    R = np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * K @ K
-    m = np.eye(4)
-    m[:3, :3] = R
-    m[:3, 3] = c["center"] + c["sway"] * np.sin(2.0 * t)
-    return m.astype(np.float32)
-
-
-def world_spheres(c, m):
-    sp = np.array(c["spheres"], np.float32)
-    out = sp.copy()
-    out[:, :3] = sp[:, :3] @ m[:3, :3].T + m[:3, 3]
-    return out
-
-
-def make_c3(ba, ctx, comp):
-    grooms = []
-    params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0, iterations=8)
-    for c in comp:
-        pos, vel, roots = ba.sphere_groom(c["S"], c["N"], radius=1.0, length=c["L"], jitter=0.05, seed=c["seed"])
-        # initial state at the groom's world placement at t = 0 (roots stay in
-        # object space: the step applies the transform)
-        m0 = rigid(c, 0.0)
-        pos[..., :3] = pos[..., :3] @ m0[:3, :3].T + m0[:3, 3]
-        g = ba.Groom(ctx, pos, vel, rest_roots=roots)
-        g.set_params(params)
-        g.set_root_mode(ba.FTL_ROOT_REST)
-        grooms.append(g)
-        del pos, vel, roots
-    return grooms
-
-
-def c3_frame_inputs(comp, grooms, t):
-    for c, g in zip(comp, grooms):
-        m = rigid(c, t)
-        g.set_root_transform(m)
-        g.set_colliders(world_spheres(c, m))
-
-
-def c3_inputs_ahead(comp, times):
-    """The per-frame inputs of the batched workload for every frame, computed
-    before the timed frames (the rigid motions are synthetic data, and their
-    numpy evaluation per groom and frame costs about as much host time as a
-    frame of GPU work): per groom, the column-major root transforms (frames,
-    16) and the world-space colliders (frames, k, 4).  The same values as
-    c3_frame_inputs up to float32 rounding of the vectorised evaluation."""
-    out = []
-    for c in comp:
-        ms = rigid_many(c, times)
-        cm = np.ascontiguousarray(ms.transpose(0, 2, 1)).reshape(len(times), 16)
-        sp = np.array(c["spheres"], np.float32)
-        ws = np.repeat(sp[None], len(times), axis=0)
-        ws[:, :, :3] = np.einsum("kj,nij->nki", sp[:, :3], ms[:, :3, :3]) + ms[:, None, :3, 3]
-        out.append((np.ascontiguousarray(cm, np.float32), np.ascontiguousarray(ws, np.float32)))
-    return out
-
-
-def c3_set_frame(ba, grooms, inputs, f):
-    """Hands frame f of c3_inputs_ahead to the grooms (two library calls per
-    groom, as c3_frame_inputs makes)."""
-    L = ba.lib()
-    for g, (cm, ws) in zip(grooms, inputs):
-        ba.check(L.ftl_groom_set_root_transform(g.handle, cm[f].ctypes.data), "ftl_groom_set_root_transform")
-        ba.check(L.ftl_groom_set_colliders(g.handle, ws[f].ctypes.data, ws.shape[1]), "ftl_groom_set_colliders")
+                       "OpenMP over strands on %d threads" % (S, N, k_all, el_all, threads),
+                single_thread_value=v_one, c1_4096x16_100_steps=c1, host=info)
 
 
 def bench_batched(ba, hipm, frames, warmup, comp=None):
-    comp = comp or c3_grooms()
+    comp = comp or wl.c3_grooms()
     ctx = context(ba)
-    grooms = make_c3(ba, ctx, comp)
+    grooms = wl.make_c3(ba, ctx, comp)
     cps = sum(c["S"] * c["N"] for c in comp)
     stream = hipm.Stream()
     e0, e1 = hipm.Event(), hipm.Event()
@@ -283,11 +216,11 @@ def bench_batched(ba, hipm, frames, warmup, comp=None):
     res = {}
     # Time runs on through both legs: the plain leg continues the motion where
     # the batched one stopped (a jump back in time would teleport every root).
-    inputs = c3_inputs_ahead(comp, (np.arange(2 * (warmup + frames)) + 1) * float(DT))
+    inputs = wl.c3_inputs_ahead(comp, (np.arange(2 * (warmup + frames)) + 1) * float(DT))
     frame0 = 0
     for mode in ("batched", "plain"):
         for k in range(frame0, frame0 + warmup):
-            c3_set_frame(ba, grooms, inputs, k)
+            wl.c3_set_frame(ba, grooms, inputs, k)
             if mode == "batched":
                 batch.launch(dts, stream.h)
             else:
@@ -297,7 +230,7 @@ def bench_batched(ba, hipm, frames, warmup, comp=None):
         w0 = time.perf_counter()
         e0.record(stream.h)
         for k in range(frame0 + warmup, frame0 + warmup + frames):
-            c3_set_frame(ba, grooms, inputs, k)
+            wl.c3_set_frame(ba, grooms, inputs, k)
             if mode == "batched":
                 batch.launch(dts, stream.h)
             else:
@@ -318,77 +251,26 @@ def bench_batched(ba, hipm, frames, warmup, comp=None):
     return dict(cps=cps, ngrooms=len(comp), ntiles=ntiles, grid=grid, **res)
 
 
-def rigid_many(c, t):
-    """rigid() for an array of times: (len(t), 4, 4) row-major."""
-    t = np.asarray(t, np.float64)
-    a = c["w"] * t
-    k = c["axis"]
-    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
-    K2 = K @ K
-    m = np.zeros((t.shape[0], 4, 4))
-    m[:, :3, :3] = np.eye(3) + np.sin(a)[:, None, None] * K + (1 - np.cos(a))[:, None, None] * K2
-    m[:, :3, 3] = c["center"] + c["sway"][None, :] * np.sin(2.0 * t)[:, None]
-    m[:, 3, 3] = 1.0
-    return m.astype(np.float32)
-
-
-def served_schedule(comp, seconds, seed, rate=60.0, jitter=0.3):
-    """Arrival times and inputs of every step request: creature i asks at a
-    Poisson rate 60 Hz x (1 + U(-0.3, 0.3)); its k-th request steps dt = 1/60
-    at simulated time (k + 1)/60 under its rigid motion."""
-    import barbu_amd as ba
-    rng = np.random.default_rng(seed)
-    rates = rate * (1.0 + rng.uniform(-jitter, jitter, size=len(comp)))
-    times, ids, reqs = [], [], []
-    for i, c in enumerate(comp):
-        n = rng.poisson(rates[i] * seconds * 1.2) + 8
-        arr = np.cumsum(rng.exponential(1.0 / rates[i], size=n))
-        arr = arr[arr < seconds]
-        if arr.size == 0:
-            continue
-        tsim = (np.arange(arr.size) + 1) * float(DT)
-        ms = rigid_many(c, tsim)
-        r = ba.make_requests(arr.size, DT)
-        r["time"] = tsim
-        r["has_transform"] = 1
-        r["model"] = np.transpose(ms, (0, 2, 1)).reshape(-1, 16)
-        sp = np.array(c["spheres"], np.float32)
-        wsp = np.einsum("nij,kj->nki", ms[:, :3, :3], sp[:, :3]) + ms[:, None, :3, 3]
-        r["ncolliders"] = sp.shape[0]
-        sph = np.zeros((arr.size, 4, 4), np.float32)
-        sph[:, : sp.shape[0], :3] = wsp
-        sph[:, : sp.shape[0], 3] = sp[:, 3]
-        r["spheres"] = sph.reshape(-1, 16)
-        times.append(arr)
-        ids.append(np.full(arr.size, i, np.int32))
-        reqs.append(r)
-    times = np.concatenate(times)
-    order = np.argsort(times, kind="stable")
-    return times[order], np.concatenate(ids)[order], np.concatenate(reqs)[order], rates
-
-
-def bench_served(ba, hipm, seconds, warm, ncreatures=256, seed=11, strand_scale=1.0):
+def bench_served(ba, hipm, seconds, warm, ncreatures=256, seed=11, strand_scale=1.0, coalesce_us=0,
+                 target_bytes=0, inflight=2):
     """Served request loop: resident creatures, Poisson step requests submitted
-    as they fall due, continuous batching (8 GB per-batch cap, 2 batches in
-    flight), latency p50/p99 from submit to completion."""
-    comp = c3_grooms(seed=seed, ngrooms=ncreatures)
-    for c in comp:
-        c["S"] = max(64, int(round(c["S"] * strand_scale)))
+    as they fall due, continuous batching (8 GB per-batch cap), latency p50/p99
+    from submit to completion."""
+    comp = wl.served_composition(ncreatures, seed, strand_scale)
     ctx = context(ba)
-    srv = ba.Server(ctx, batch_cap_bytes=8 << 30, max_inflight=2)
-    params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0, iterations=8)
+    srv = ba.Server(ctx, batch_cap_bytes=8 << 30, max_inflight=inflight, coalesce_us=coalesce_us,
+                    batch_target_bytes=target_bytes)
+    params = wl.c3_params(ba)
     sid = []
     for c in comp:
-        pos, vel, roots = ba.sphere_groom(c["S"], c["N"], radius=1.0, length=c["L"], jitter=0.05, seed=c["seed"])
-        m0 = rigid(c, 0.0)
-        pos[..., :3] = pos[..., :3] @ m0[:3, :3].T + m0[:3, 3]
+        pos, vel, roots = wl.c3_state(ba, c)
         cid, resident = srv.add(pos, vel, rest_roots=roots, params=params)
         if not resident:
             raise RuntimeError("served workload does not fit the HBM budget")
         sid.append(cid)
         del pos, vel, roots
     cps = sum(c["S"] * c["N"] for c in comp)
-    times, idx, reqs, rates = served_schedule(comp, warm + seconds, seed)
+    times, idx, reqs, rates = wl.served_schedule(ba, comp, warm + seconds, seed)
     ids = np.asarray(sid, np.int32)[idx]
     n = times.shape[0]
     t0 = time.perf_counter()
@@ -409,29 +291,24 @@ def bench_served(ba, hipm, seconds, warm, ncreatures=256, seed=11, strand_scale=
     st = srv.stats()
     # A strand whose consecutive points meet (two points pushed onto the same
     # spot of a collider) divides by a zero length, as normalize(0) in the
-    # shader does; the oracle reproduces such strands bit for bit.  Counted,
-    # not hidden.
+    # shader does; the oracle reproduces such strands bit for bit.  The server
+    # quarantines such a creature; counted, not hidden.
     bad = int(sum(srv.status(c) for c in sid))
     srv.close()
     ctx.close()
+    span = st["span_s"]
     return dict(creatures=len(comp), control_points=cps, requests=int(st["completed"]),
-                nonfinite_creatures=bad,
+                quarantined_creatures=bad, dropped_requests=int(st["dropped"]),
                 rate_hz_mean=float(np.mean(rates)), seconds=seconds, warmup_s=warm,
+                coalesce_us=coalesce_us, batch_target_bytes=target_bytes, max_inflight=inflight,
                 p50_ms=st["p50_us"] / 1000.0, p99_ms=st["p99_us"] / 1000.0, max_ms=st["max_us"] / 1000.0,
                 mean_ms=st["mean_us"] / 1000.0, batches=int(st["batches"]),
                 mean_batch_requests=st["mean_batch_requests"],
-                value=st["cps_stepped"] / st["span_s"] if st["span_s"] > 0 else 0.0, unit="CP updates/s",
+                value=st["cps_stepped"] / span if span > 0 else 0.0, unit="CP updates/s",
+                value_finite=st["cps_finite"] / span if span > 0 else 0.0,
+                per_busy_second=st["cps_stepped"] / st["busy_s"] if st["busy_s"] > 0 else 0.0,
                 demand=float(np.sum(rates * np.array([c["S"] * c["N"] for c in comp]))),
-                device_busy_frac=st["busy_s"] / st["span_s"] if st["span_s"] > 0 else 0.0)
-
-
-def c5_spheres(t):
-    """Four colliders of the skinned workload at time t: the head (r=1 at the
-    origin) and three spheres orbiting it."""
-    return np.array([[0.0, 0.0, 0.0, 1.0],
-                     [1.3 * np.cos(t), 0.2, 1.3 * np.sin(t), 0.35],
-                     [-1.2, -0.5 + 0.2 * np.sin(2 * t), 0.3, 0.4],
-                     [0.3 * np.sin(t), 1.25, -1.1 * np.cos(t), 0.3]], np.float32)
+                device_busy_frac=st["busy_s"] / span if span > 0 else 0.0)
 
 
 def bench_skinned(ba, hipm, steps, warmup, S=4_194_304, N=32):
@@ -444,36 +321,43 @@ def bench_skinned(ba, hipm, steps, warmup, S=4_194_304, N=32):
     skin = ba.Skin(ctx, sc["rest"], sc["joints"], sc["weights"], sc["tris"], sc["nbones"])
     g = ba.Groom(ctx, sc["pos"], sc["vel"])
     g.bind_skin(skin, sc["tri"], sc["bary"])
-    g.set_params(ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0, iterations=8))
+    g.set_params(wl.c5_params(ba))
     nverts = sc["rest"].shape[0]
     del sc["pos"], sc["vel"]
     stream = hipm.Stream()
     e0, e1 = hipm.Event(), hipm.Event()
+    bones = [ba.bones(sc, (k + 1) * float(DT)) for k in range(warmup + steps)]
+    sphs = [wl.c5_spheres((k + 1) * float(DT)) for k in range(warmup + steps)]
+    ks = [(hipm.Event(), hipm.Event()) for _ in range(steps)]
 
-    def frame(k):
-        t = (k + 1) * float(DT)
-        skin.set_bones(ba.bones(sc, t))
+    def frame(k, ev=None):
+        skin.set_bones(bones[k])
         skin.update(stream.h)
-        g.set_colliders(c5_spheres(t))
-        g.step(DT, stream.h)
+        g.set_colliders(sphs[k])
+        if ev is None:
+            g.step(DT, stream.h)
+        else:
+            g.step_timed(DT, stream.h, ev[0].h, ev[1].h)
 
     for k in range(warmup):
         frame(k)
     hipm.device_synchronize()
     w0 = time.perf_counter()
     e0.record(stream.h)
-    for k in range(warmup, warmup + steps):
-        frame(k)
+    for i, k in enumerate(range(warmup, warmup + steps)):
+        frame(k, ks[i])
     e1.record(stream.h)
     hipm.device_synchronize()
     wall = time.perf_counter() - w0
     gpu_ms = e0.elapsed_ms(e1)
+    kern_ms = np.array([a.elapsed_ms(b) for a, b in ks])
     nonfinite = g.status()
     lanes = ctx.lanes_for(N)
     g.close()
     skin.close()
     ctx.close()
-    return dict(S=S, N=N, nverts=nverts, wall_s=wall, gpu_ms=gpu_ms, lanes=lanes, nonfinite=bool(nonfinite))
+    return dict(S=S, N=N, nverts=nverts, wall_s=wall, gpu_ms=gpu_ms, lanes=lanes, nonfinite=bool(nonfinite),
+                kernel_us_mean=float(kern_ms.mean() * 1000.0))
 
 
 def bench_expand(ba, hipm, reps, warmup, ntris=100_000, npoints=4, seed=5):
@@ -560,6 +444,8 @@ def load_traffic(workload):
         return None
 
 
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -570,6 +456,10 @@ def main():
     ap.add_argument("--workload", default="all",
                     choices=["single", "batched", "served", "skinned", "expand", "marschner", "all"])
     ap.add_argument("--served-seconds", type=float, default=4.0)
+    ap.add_argument("--served-coalesce-us", type=int, default=1000,
+                    help="coalescing window of the served loop (0: launch a batch as soon as a slot frees)")
+    ap.add_argument("--served-target-mb", type=int, default=2048,
+                    help="traffic at which a coalescing batch launches at once, MB")
     ap.add_argument("--skinned-steps", type=int, default=120)
     ap.add_argument("--batched-frames", type=int, default=60)
     ap.add_argument("--lanes", type=int, default=0, choices=[0, 1, 2, 4],
@@ -583,16 +473,13 @@ def main():
     import barbu_amd as ba
     from barbu_amd import _hip as hipm
 
-    if args.workload == "batched":
-        args.steps = max(args.steps, 1)
     log("single groom: 262144 x 32, %d warmup + %d timed steps" % (args.warmup, args.steps))
     r = bench_single(ba, hipm, args.steps, args.warmup, lanes=args.lanes or None)
     cps = r["S"] * r["N"]
     ms_per_step = 1000.0 * r["wall_s"] / args.steps
     value = cps * args.steps / r["wall_s"]
-    launch_s = r["gpu_ms"] / 1000.0 / args.steps
-    achieved = cps * BYTES_PER_CP / launch_s / 1e9
-    traffic = load_traffic("single")
+    kernel_s = r["kernel_us_mean"] * 1e-6
+    achieved = cps * BYTES_PER_CP / kernel_s / 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -611,8 +498,12 @@ def main():
                                "at 1.2, dt=1/60, 8 iterations, damping 0.9",
                    "strands": r["S"], "points": r["N"], "lanes_per_strand": r["lanes"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_us": launch_s * 1e6, "bytes_per_launch": cps * BYTES_PER_CP,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("single"),
+                     "kernel": "ftl_plain_wind_kernel",
+                     "kernel_us": r["kernel_us_mean"], "kernel_us_median": r["kernel_us_median"],
+                     "kernel_us_min": r["kernel_us_min"], "kernel_us_max": r["kernel_us_max"],
+                     "stream_window_us": r["gpu_ms"] * 1000.0 / args.steps,
+                     "bytes_per_launch": cps * BYTES_PER_CP,
                      "copy_reference_gbs": r["copy_gbs"]},
     }
     if args.workload in ("batched", "all"):
@@ -623,43 +514,55 @@ def main():
         fp = b["plain"]
         v_b = b["cps"] * args.batched_frames / fb["wall_s"]
         v_p = b["cps"] * args.batched_frames / fp["wall_s"]
-        gb = b["cps"] * BYTES_PER_CP / (fb["gpu_ms"] / 1000.0 / args.batched_frames) / 1e9
+        frame_s = fb["gpu_ms"] / 1000.0 / args.batched_frames
+        gb = b["cps"] * BYTES_PER_CP / frame_s / 1e9
         out["batched"] = {
             "workload": "64 grooms, strands log-uniform [1024, 262144], points {8,16,32,64}, lengths U(0.05,0.4), "
                         "random rigid motion, 1-3 spheres; one graph launch per frame",
             "control_points": b["cps"], "tiles": b["ntiles"], "grid": b["grid"], "frames": args.batched_frames,
             "value": v_b, "unit": "CP updates/s", "ms_per_frame": 1000.0 * fb["wall_s"] / args.batched_frames,
             "gpu_ms_per_frame": fb["gpu_ms"] / args.batched_frames,
-            "achieved_gbs": gb, "frac_hbm_peak": gb / HBM_PEAK_GBS,
             "plain_value": v_p, "plain_ms_per_frame": 1000.0 * fp["wall_s"] / args.batched_frames,
+            "plain_gpu_ms_per_frame": fp["gpu_ms"] / args.batched_frames,
             "speedup_vs_plain": v_b / v_p,
+            "roofline": {"bound": "hbm", "achieved": gb, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gb / HBM_PEAK_GBS, "traffic": load_traffic("batched"),
+                         "frame_us": frame_s * 1e6, "bytes_per_frame": b["cps"] * BYTES_PER_CP,
+                         "timing": "HIP events around the timed graph launches on their stream; a frame is "
+                                   "the descriptor upload plus one tile-list kernel per launch class, the "
+                                   "classes running side by side"},
         }
     if args.workload in ("served", "all"):
-        log("served: 256 creatures (sizes as the batched workload), Poisson requests at 60 Hz +-30%%, %.1f s"
-            % args.served_seconds)
-        sv = bench_served(ba, hipm, args.served_seconds, 1.0)
+        target = args.served_target_mb << 20
+        log("served: 256 creatures (sizes as the batched workload), Poisson requests at 60 Hz +-30%%, %.1f s, "
+            "coalescing %d us" % (args.served_seconds, args.served_coalesce_us))
+        sv = bench_served(ba, hipm, args.served_seconds, 1.0, coalesce_us=args.served_coalesce_us,
+                          target_bytes=target)
         sv["workload"] = ("256 resident creatures, sizes drawn as the batched workload (strands log-uniform "
                           "[1024, 262144], points {8,16,32,64}); requests Poisson per creature at 60 Hz x "
                           "(1 + U(-0.3, 0.3)); 8 GB per-batch cap; 2 batches in flight")
         out["served"] = sv
         log("served at the stated total (about 20M control points: strand counts scaled by 1/18)")
-        sv20 = bench_served(ba, hipm, args.served_seconds, 1.0, strand_scale=1.0 / 18.0)
+        sv20 = bench_served(ba, hipm, args.served_seconds, 1.0, strand_scale=1.0 / 18.0,
+                            coalesce_us=args.served_coalesce_us, target_bytes=target)
         sv20["workload"] = "as served, strand counts scaled by 1/18 (about 20M control points)"
         out["served_20m"] = sv20
     if args.workload in ("skinned", "all"):
         log("skinned: 4,194,304 x 32, 100k-triangle mesh, 64 bones, 4 colliders, %d frames" % args.skinned_steps)
         k = bench_skinned(ba, hipm, args.skinned_steps, 5)
         kcps = k["S"] * k["N"]
-        kgb = kcps * BYTES_PER_CP / (k["gpu_ms"] / 1000.0 / args.skinned_steps) / 1e9
+        kgb = kcps * BYTES_PER_CP / (k["kernel_us_mean"] * 1e-6) / 1e9
         out["skinned"] = {
             "workload": "4,194,304 strands x 32 points, roots bound to a 100k-triangle mesh skinned by 64 bones "
                         "(4 influences, dual quaternions, rotations <= 30 deg), 4 colliders, dt=1/60",
             "control_points": kcps, "mesh_vertices": k["nverts"], "frames": args.skinned_steps,
             "value": kcps * args.skinned_steps / k["wall_s"], "unit": "CP updates/s",
             "ms_per_frame": 1000.0 * k["wall_s"] / args.skinned_steps,
-            "gpu_ms_per_frame": k["gpu_ms"] / args.skinned_steps,
-            "achieved_gbs": kgb, "frac_hbm_peak": kgb / HBM_PEAK_GBS, "lanes_per_strand": k["lanes"],
+            "gpu_ms_per_frame": k["gpu_ms"] / args.skinned_steps, "lanes_per_strand": k["lanes"],
             "nonfinite": k["nonfinite"],
+            "roofline": {"bound": "hbm", "achieved": kgb, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": kgb / HBM_PEAK_GBS, "traffic": load_traffic("skinned"),
+                         "kernel": "ftl_plain_kernel", "kernel_us": k["kernel_us_mean"]},
         }
     if args.workload in ("expand", "all"):
         log("render-side expansion: 100k-triangle scalp, 4-point guides, 3 x 2 x 16 tessellation")

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FTL_ABI_VERSION 1
+#define FTL_ABI_VERSION 2
 #define FTL_MAX_COLLIDERS 4
 #define FTL_MAX_POINTS 64
 #define FTL_WIND_GRID 8    /* side of the periodic wind lattice */
@@ -167,6 +167,12 @@ ftl_status ftl_groom_set_time(ftl_groom* g, double seconds);
 /* One simulation step of dt seconds, enqueued on `stream`.  ≙ Hair::update
    (hair.cc:89-125), without the copy-back.  Asynchronous. */
 ftl_status ftl_groom_step(ftl_groom* g, float dt, void* stream);
+/* ftl_groom_step with its kernel bracketed by two caller-created hipEvent_t
+   (passed as void*) that take the kernel's own start and end timestamps
+   (hipExtLaunchKernelGGL): hipEventElapsedTime(start, stop) is the launch's
+   duration without host or queue gaps.  Diagnostics; no counterpart in the
+   reference. */
+ftl_status ftl_groom_step_timed(ftl_groom* g, float dt, void* stream, void* start_event, void* stop_event);
 
 /* Device pointers of the state (float4 arrays above), for zero-copy consumers
    such as a renderer (the reference's VAO reads them, hair.cc:363-391). */
@@ -197,20 +203,67 @@ typedef struct ftl_skin_desc {
   const int* triangles;         /* int3 [ntris] vertex indices                  */
 } ftl_skin_desc;
 
+/* Blending of the skinning stage: the uSkinning subroutine of
+   inc_skinning.glsl:15-16 (SkinComponent's SkinningMode, skin.cc:63-74). */
+enum {
+  FTL_SKIN_DQ = 0,  /* dual-quaternion blending, skinning_DQBS (:54-82) */
+  FTL_SKIN_LBS = 1  /* linear blending of 3x4 joint matrices, skinning_LBS (:85-129) */
+};
+/* Joint index meaning "no joint": the identity matrix in linear blending
+   (kNoJoint, inc_skinning.glsl:18,89-93). */
+#define FTL_NO_JOINT 0xff
+
+/* Joint indices must be in [0, nbones), or FTL_NO_JOINT (linear blending
+   only).  A new skin blends dual quaternions, with identity bones. */
 ftl_status ftl_skin_create(ftl_ctx* ctx, const ftl_skin_desc* desc, ftl_skin** out);
 void ftl_skin_destroy(ftl_skin* s);
+/* FTL_SKIN_DQ or FTL_SKIN_LBS (dual quaternions are refused while a joint
+   index is FTL_NO_JOINT). */
+ftl_status ftl_skin_set_mode(ftl_skin* s, int mode);
 /* Bone dual quaternions for the next skinning: 8 floats per bone, real part
    xyzw then dual part xyzw (glm::dualquat memory order, uploaded as a float4
    texture buffer by SkinComponent::updateSkinningBuffer, skin.cc:46-84). */
 ftl_status ftl_skin_set_bones(ftl_skin* s, const float* dq, int nbones);
-/* Skins every vertex (inc_skinning.glsl:22-31,54-82) on `stream`. */
+/* Bone skinning matrices for the next skinning, in the layout the reference
+   uploads for linear blending: 12 floats per bone, the three rows of the
+   affine skinning matrix (glm::mat3x4(transpose(skin_matrix)),
+   skeleton_controller.cc:251-256).  Also sets the bones' dual quaternions,
+   converted from the same matrices (glm::dualquat(mat3x4),
+   skeleton_controller.cc:259-264), so either mode can be used. */
+ftl_status ftl_skin_set_matrices(ftl_skin* s, const float* m3x4, int nbones);
+/* SkeletonController::generate_skinning_datas (skeleton_controller.cc:248-265):
+   skin_i = global_pose_i * inverse_bind_i (column-major 4x4 each, glm::mat4
+   memory order; inverse_bind may be NULL for identities), then as
+   ftl_skin_set_matrices. */
+ftl_status ftl_skin_set_joint_matrices(ftl_skin* s, const float* global_pose, const float* inverse_bind,
+                                       int nbones);
+/* Rest normals (float4 [nverts]) skinned with the positions from now on:
+   rotated by the blended real part (:80-81), or blended as (n, 0) by the
+   joint matrices (:128).  Not renormalised, as in the reference. */
+ftl_status ftl_skin_set_rest_normals(ftl_skin* s, const float* normals);
+/* Skins every vertex (and normal) on `stream` (inc_skinning.glsl:22-31 and
+   the mode's subroutine). */
 ftl_status ftl_skin_update(ftl_skin* s, void* stream);
 /* Synchronous copy of the skinned vertices (float4 [nverts]). */
 ftl_status ftl_skin_download(ftl_skin* s, float* verts);
+/* Synchronous copy of the skinned normals (float4 [nverts]); needs rest normals. */
+ftl_status ftl_skin_download_normals(ftl_skin* s, float* normals);
+/* The dual quaternion of a rigid 3x4 transform (12 floats: rows of [R | t]):
+   real part q = the rotation's quaternion, dual part = 0.5 (t, 0) q; 8
+   floats as ftl_skin_set_bones.  Host only.  glm::dualquat(mat3x4). */
+ftl_status ftl_dualquat_from_matrix(const float* m3x4, float* dq);
 /* Binds each strand's root to a triangle: tri[s] in [0,ntris), bary[2s..2s+1]
    = (w1, w2) with w0 = 1 - w1 - w2.  Sets the groom's root mode to
    FTL_ROOT_SKINNED.  The skin must outlive the binding. */
 ftl_status ftl_groom_bind_skin(ftl_groom* g, ftl_skin* s, const int* tri, const float* bary);
+/* The emitter frame of each strand of a skinned groom at the skin's current
+   pose: its root on the bound triangle (the root the next step uses) and,
+   when the skin has normals, the unit normal there, normalize(w0 nA + w1 nB
+   + w2 nC) (float4 [nstrands] each, w = 0; normals may be NULL).  These are
+   the roots and normals Hair::init_simulation grows strands from
+   (hair.cc:258-288; ftl_groom_create_reference takes them), i.e. the "post
+   skinning" roots cs_simulation.glsl:186-188 asks for.  Synchronous. */
+ftl_status ftl_groom_root_frames(ftl_groom* g, float* roots, float* normals);
 
 /* -- batched steps --------------------------------------------------------- */
 

@@ -41,6 +41,13 @@ typedef struct ftl_server_config {
   uint64_t hbm_budget_bytes; /* device bytes resident creatures may hold; 0 = 80% of free device memory */
   uint64_t batch_cap_bytes;  /* HBM traffic per batch (64 B per control point); 0 = 8 GB */
   int max_inflight;          /* batches on the device at once, 1..4; 0 = 2 */
+  /* Coalescing window, microseconds (0 = none).  While another batch is on
+     the device, a new batch is launched only once its requests reach
+     batch_target_bytes of traffic or its oldest request has waited
+     coalesce_us: under load, fewer and larger batches, traded against
+     latency.  With no batch on the device a batch always launches at once. */
+  int coalesce_us;
+  uint64_t batch_target_bytes; /* 0 = the per-batch cap */
 } ftl_server_config;
 
 /* Inputs of one step of one creature (the uniforms Hair::update sets before
@@ -66,6 +73,10 @@ typedef struct ftl_server_stats {
   uint64_t resident_bytes;
   int resident;           /* creatures on the device */
   int waiting;            /* creatures waiting for memory */
+  uint64_t cps_finite;    /* control-point updates of completed steps that stayed finite */
+  uint64_t dropped;       /* requests completed without a step (quarantined creature) */
+  uint64_t failed_admissions; /* waiting creatures dropped because they could not be made resident */
+  int quarantined;        /* creatures quarantined now */
 } ftl_server_stats;
 
 ftl_status ftl_server_create(ftl_ctx* ctx, const ftl_server_config* cfg, ftl_server** out);
@@ -99,8 +110,15 @@ ftl_status ftl_server_wait(ftl_server* s, uint64_t ticket, int timeout_ms);
 ftl_status ftl_server_drain(ftl_server* s, int timeout_ms);
 /* Copies a creature's state to the host (waits for its step in flight). */
 ftl_status ftl_server_download(ftl_server* s, int id, float* pos, float* vel);
-/* Non-finite flag of the creature's steps since the last call (cleared). */
+/* 1 when the creature is quarantined: one of its steps left NaN/Inf at a
+   strand tip (the reference's normalize(0) on a zero-length segment,
+   cs_simulation.glsl:114, propagates that way).  A quarantined creature is
+   left out of batches; its queued and later requests complete without a step
+   and ftl_server_wait reports them with FTL_ERR_NONFINITE. */
 ftl_status ftl_server_status(ftl_server* s, int id, int* nonfinite);
+/* Replaces a creature's state (float4 arrays as ftl_groom_upload; waits for
+   its step in flight) and lifts its quarantine. */
+ftl_status ftl_server_upload(ftl_server* s, int id, const float* pos, const float* vel);
 
 ftl_status ftl_server_get_stats(ftl_server* s, ftl_server_stats* out);
 /* Clears the latency samples and counters (not the creatures); requests

@@ -17,7 +17,9 @@
 //   src/shaders/hair/interop.h:8-30        (float4 pos with w = rest length, float4 vel)
 //   src/fx/hair.cc:89-125                  (one update = one dispatch; the next
 //                                            step reads this step's output)
-//   src/shaders/shared/inc_skinning.glsl:22-82  (dual-quaternion blending, roots)
+//   src/shaders/shared/inc_skinning.glsl:22-129 (dual-quaternion and linear
+//                                          blending of positions and normals,
+//                                          kNoJoint; skinned roots)
 //   src/shaders/hair/02_tess_stream/{vs,tcs,tes,gs}_stream_hair.glsl and
 //   src/shaders/shared/inc_maths.glsl:107-115,210-228,239-241,263-266
 //                                          (render-side strand expansion, the
@@ -347,13 +349,17 @@ T dot4(const T* a, const T* b) {
   return std::fma(a[3], b[3], std::fma(a[2], b[2], std::fma(a[1], b[1], a[0] * b[0])));
 }
 
+// skinning_DQBS (inc_skinning.glsl:54-82) after apply_skinning's weight rule
+// (:22-31).  normal / nout (optional, float4): the normal is rotated by the
+// blended real part (:80-81), not renormalised.
 template <typename T>
 void skin_vertex(const float* dq, const int* joints, const float* weights, const float* rest,
-                 float* out) {
+                 float* out, const float* normal = nullptr, float* nout = nullptr) {
   V3<T> v = {(T)rest[0], (T)rest[1], (T)rest[2]};
   T w[4] = {(T)weights[0], (T)weights[1], (T)weights[2], T(0)};
   if (w[0] <= T(1e-6f)) {  // Epsilon() (inc_constants.glsl:6)
     out[0] = (float)v.x; out[1] = (float)v.y; out[2] = (float)v.z; out[3] = rest[3];
+    if (nout) { nout[0] = normal[0]; nout[1] = normal[1]; nout[2] = normal[2]; nout[3] = normal[3]; }
     return;
   }
   w[3] = T(1) - (w[0] + w[1] + w[2]);
@@ -388,6 +394,53 @@ void skin_vertex(const float* dq, const int* joints, const float* weights, const
                     std::fma(A[3], b.z, std::fma(-B[3], a.z, ab.z))};
   v = {std::fma(T(2), tr.x, v.x), std::fma(T(2), tr.y, v.y), std::fma(T(2), tr.z, v.z)};
   out[0] = (float)v.x; out[1] = (float)v.y; out[2] = (float)v.z; out[3] = rest[3];
+  if (nout) {
+    // n += 2 cross(a, cross(a, n) + a.w n), the rotation above
+    V3<T> n = {(T)normal[0], (T)normal[1], (T)normal[2]};
+    const V3<T> cn = cross_t(a, n);
+    const V3<T> tn = {std::fma(A[3], n.x, cn.x), std::fma(A[3], n.y, cn.y), std::fma(A[3], n.z, cn.z)};
+    const V3<T> rn = cross_t(a, tn);
+    n = {std::fma(T(2), rn.x, n.x), std::fma(T(2), rn.y, n.y), std::fma(T(2), rn.z, n.z)};
+    nout[0] = (float)n.x; nout[1] = (float)n.y; nout[2] = (float)n.z; nout[3] = normal[3];
+  }
+}
+
+// skinning_LBS (inc_skinning.glsl:85-129) after apply_skinning's weight rule.
+// mats: 12 floats per bone, the three rows of its skinning matrix (the
+// transposed 3x4 the host uploads, skeleton_controller.cc:251-256); a joint
+// index of 0xff (kNoJoint, :18) is the identity (:89-93).  Each bone's
+// transform of (v, 1) and of (n, 0) is dot(vertex, row c) evaluated as
+// fma(x3, r3, fma(x2, r2, fma(x1, r1, x0 * r0))), then blended as
+// fma(w3, t3, fma(w2, t2, fma(w1, t1, w0 * t0))) (M * _weights, :117).
+template <typename T>
+void lbs_vertex(const float* mats, const int* joints, const float* weights, const float* rest, float* out,
+                const float* normal = nullptr, float* nout = nullptr) {
+  T w[4] = {(T)weights[0], (T)weights[1], (T)weights[2], T(0)};
+  if (w[0] <= T(1e-6f)) {
+    for (int c = 0; c < 4; ++c) out[c] = rest[c];
+    if (nout)
+      for (int c = 0; c < 4; ++c) nout[c] = normal[c];
+    return;
+  }
+  w[3] = T(1) - (w[0] + w[1] + w[2]);
+  T R[4][3][4];
+  for (int k = 0; k < 4; ++k)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 4; ++c)
+        R[k][r][c] = joints[k] == 0xff ? T(r == c ? 1 : 0) : (T)mats[12 * joints[k] + 4 * r + c];
+  auto apply = [&](const float* x, T hw, float* o, float ow) {
+    const T xv[4] = {(T)x[0], (T)x[1], (T)x[2], hw};
+    T t[4][3];
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r)
+        t[k][r] = std::fma(xv[3], R[k][r][3], std::fma(xv[2], R[k][r][2], std::fma(xv[1], R[k][r][1],
+                                                                                   xv[0] * R[k][r][0])));
+    for (int r = 0; r < 3; ++r)
+      o[r] = (float)std::fma(w[3], t[3][r], std::fma(w[2], t[2][r], std::fma(w[1], t[1][r], w[0] * t[0][r])));
+    o[3] = ow;
+  };
+  apply(rest, T(1), out, rest[3]);
+  if (nout) apply(normal, T(0), nout, normal[3]);
 }
 
 }  // namespace
@@ -433,6 +486,54 @@ void oracle_skin_f64(int nverts, const float* dq, const int* joints, const float
                      const float* rest, float* out_as_f32) {
   for (int v = 0; v < nverts; ++v)
     skin_vertex<double>(dq, joints + 4 * v, weights + 4 * v, rest + 4 * v, out_as_f32 + 4 * v);
+}
+
+// Positions and normals (float4 each) of the dual-quaternion (mode 0) or
+// linear-blend (mode 1, bones as 3x4 row matrices, 12 floats each) skinning.
+// FP64 variant: the same formulas in double, rounded to float at the end.
+void oracle_skin_full_f32(int mode, int nverts, const float* bones, const int* joints, const float* weights,
+                          const float* rest, const float* normals, float* out, float* nout) {
+  for (int v = 0; v < nverts; ++v) {
+    if (mode == 0)
+      skin_vertex<float>(bones, joints + 4 * v, weights + 4 * v, rest + 4 * v, out + 4 * v, normals + 4 * v,
+                         nout + 4 * v);
+    else
+      lbs_vertex<float>(bones, joints + 4 * v, weights + 4 * v, rest + 4 * v, out + 4 * v, normals + 4 * v,
+                        nout + 4 * v);
+  }
+}
+void oracle_skin_full_f64(int mode, int nverts, const float* bones, const int* joints, const float* weights,
+                          const float* rest, const float* normals, float* out, float* nout) {
+  for (int v = 0; v < nverts; ++v) {
+    if (mode == 0)
+      skin_vertex<double>(bones, joints + 4 * v, weights + 4 * v, rest + 4 * v, out + 4 * v, normals + 4 * v,
+                          nout + 4 * v);
+    else
+      lbs_vertex<double>(bones, joints + 4 * v, weights + 4 * v, rest + 4 * v, out + 4 * v, normals + 4 * v,
+                         nout + 4 * v);
+  }
+}
+
+// Root frame of each strand on its bound triangle: the root as
+// oracle_bary_roots_f32, and the normal = normalize(w0 nA + w1 nB + w2 nC)
+// with the same fma order and normalize(d) = d * rsqrt(dot(d, d)).
+void oracle_root_frames_f32(int nstrands, const int* tris, const int* bind_tri, const float* bind_w,
+                            const float* verts, const float* normals, float* out_roots, float* out_normals) {
+  for (int s = 0; s < nstrands; ++s) {
+    const int t = bind_tri[s];
+    const float w1 = bind_w[2 * s], w2 = bind_w[2 * s + 1];
+    const float w0 = (1.0f - w1) - w2;
+    const int ia = tris[3 * t + 0], ib = tris[3 * t + 1], ic = tris[3 * t + 2];
+    float n[3];
+    for (int c = 0; c < 3; ++c) {
+      out_roots[4 * s + c] = std::fma(w2, verts[4 * ic + c], std::fma(w1, verts[4 * ib + c], w0 * verts[4 * ia + c]));
+      n[c] = std::fma(w2, normals[4 * ic + c], std::fma(w1, normals[4 * ib + c], w0 * normals[4 * ia + c]));
+    }
+    out_roots[4 * s + 3] = 0.0f;
+    const float is = rsqrt_t(dot3<float>({n[0], n[1], n[2]}, {n[0], n[1], n[2]}));
+    for (int c = 0; c < 3; ++c) out_normals[4 * s + c] = n[c] * is;
+    out_normals[4 * s + 3] = 0.0f;
+  }
 }
 
 // Root of each strand from its bound triangle: w0*A + w1*B + w2*C with

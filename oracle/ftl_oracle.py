@@ -60,6 +60,9 @@ def lib():
         L.oracle_skin_f32.argtypes = [C.c_int, vp, vp, vp, vp, vp]
         L.oracle_skin_f64.argtypes = [C.c_int, vp, vp, vp, vp, vp]
         L.oracle_bary_roots_f32.argtypes = [C.c_int, vp, vp, vp, vp, vp]
+        for n in ("oracle_skin_full_f32", "oracle_skin_full_f64"):
+            getattr(L, n).argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_root_frames_f32.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_max_threads.restype = C.c_int
         for n in ("oracle_expand_f32", "oracle_expand_f64"):
             getattr(L, n).restype = C.c_int
@@ -191,6 +194,39 @@ def skin_f64(dq, joints, weights, rest):
     out = np.empty_like(rest)
     lib().oracle_skin_f64(rest.shape[0], _p(dq), _p(joints), _p(weights), _p(rest), _p(out))
     return out
+
+
+SKIN_DQ, SKIN_LBS = 0, 1
+
+
+def skin_full(mode, bones, joints, weights, rest, normals, dtype=np.float32):
+    """Skinned positions and normals (float32 [nverts, 4] each): mode 0 dual
+    quaternions (8 floats per bone), mode 1 linear blending (12 floats per
+    bone, the rows of its 3x4 matrix; joint 0xff = identity).  dtype
+    float64 evaluates the same formulas in double."""
+    bones = np.ascontiguousarray(bones, np.float32)
+    joints = np.ascontiguousarray(joints, np.int32)
+    weights = np.ascontiguousarray(weights, np.float32)
+    rest = np.ascontiguousarray(rest, np.float32)
+    normals = np.ascontiguousarray(normals, np.float32)
+    out = np.empty_like(rest)
+    nout = np.empty_like(normals)
+    fn = lib().oracle_skin_full_f64 if dtype == np.float64 else lib().oracle_skin_full_f32
+    fn(int(mode), rest.shape[0], _p(bones), _p(joints), _p(weights), _p(rest), _p(normals), _p(out), _p(nout))
+    return out, nout
+
+
+def root_frames_f32(tris, tri, bary, verts, normals):
+    """Each strand's root and unit normal on its bound triangle: (roots, normals), float32 [S, 4]."""
+    tris = np.ascontiguousarray(tris, np.int32)
+    tri = np.ascontiguousarray(tri, np.int32)
+    bary = np.ascontiguousarray(bary, np.float32)
+    verts = np.ascontiguousarray(verts, np.float32)
+    normals = np.ascontiguousarray(normals, np.float32)
+    r = np.empty((tri.shape[0], 4), np.float32)
+    n = np.empty((tri.shape[0], 4), np.float32)
+    lib().oracle_root_frames_f32(tri.shape[0], _p(tris), _p(tri), _p(bary), _p(verts), _p(normals), _p(r), _p(n))
+    return r, n
 
 
 def bary_roots_f32(tris, tri, bary, verts):

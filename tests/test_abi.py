@@ -37,7 +37,7 @@ def test_ctypes_signatures_cover_the_headers():
 
 
 def test_abi_version_and_reference_params():
-    assert ba.lib().ftl_abi_version() == 1
+    assert ba.lib().ftl_abi_version() == 2
     p = ba.reference_params()
     assert tuple(p.gravity) == (0.0, np.float32(-9.81), 0.0)
     assert p.force_scale == 20.0 and p.iterations == 8

@@ -423,3 +423,51 @@ def test_tuned_lane_tables():
         assert c.lanes_for(20) == 2 and c.lanes_for(63) == 4  # pipelined guarded classes
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("npoints,lanes", [(32, 1), (32, 2), (64, 4), (20, 0), (20, 1), (40, 0), (63, 0)])
+def test_negative_length_scale_colliders(ctx, npoints, lanes):
+    """A negative length scale places each point |sigma * l| from its leader
+    on the other side: the per-wave collider cull must bound the chain by the
+    absolute lengths.  A sphere within the strands' reach, exact and guarded
+    classes, 1, 2 and 4 lanes per strand."""
+    pos, vel, _ = ba.sphere_groom(700, npoints, radius=1.0, length=0.3, seed=70 + npoints)
+    params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=-1.0)
+    base = pos[5, 0, :3].astype(np.float64)
+    d = base / np.linalg.norm(base)
+    # inside the chain's reach from the roots, on both sides of the surface
+    spheres = [(0, 0, 0, 1.0), (*(base - d * 0.2), 0.15), (*(base + d * 0.15), 0.1)]
+    run_compare(ctx, pos, vel, params, spheres, 3, lanes=lanes, label="sigma=-1 N=%d G=%d" % (npoints, lanes))
+
+
+def test_batch_step_alternating_streams(ctx):
+    """Two batches of differently shaped grooms stepped alternately on two
+    streams without host synchronisation between them: each batch's
+    descriptors must stay intact until its kernels have read them."""
+    from barbu_amd import _hip as hipm
+    sa, sb = hipm.Stream(), hipm.Stream()
+    ga, sta = _mixed_grooms(ctx, seed=21)
+    shapes_b = [(3000, 64), (5000, 8), (77, 33), (4096, 32)]
+    gb, stb = [], []
+    for i, (S, N) in enumerate(shapes_b):
+        pos, vel, roots = ba.sphere_groom(S, N, length=0.2, seed=300 + i)
+        g = ba.Groom(ctx, pos, vel, rest_roots=roots)
+        params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0)
+        g.set_params(params)
+        spheres = np.array([[0, 0, 0, 1.0]], np.float32)
+        g.set_colliders(spheres)
+        g.set_root_mode(ba.FTL_ROOT_REST)
+        gb.append(g)
+        stb.append(dict(pos=pos.copy(), vel=vel.copy(), roots=roots, params=params, spheres=spheres, N=N))
+    for k in range(3):
+        ba.batch_step(ctx, ga, [DT] * len(ga), sa.h)
+        ba.batch_step(ctx, gb, [DT] * len(gb), sb.h)
+        _oracle_frame(sta, [DT] * len(ga), [None] * len(ga))
+        _oracle_frame(stb, [DT] * len(gb), [None] * len(gb))
+    sa.synchronize()
+    sb.synchronize()
+    for name, grooms, states in (("a", ga, sta), ("b", gb, stb)):
+        for i, (g, s) in enumerate(zip(grooms, states)):
+            p, v = g.download()
+            assert_bitexact(p, s["pos"], "stream %s groom %d pos" % (name, i))
+            assert_bitexact(v, s["vel"], "stream %s groom %d vel" % (name, i))

@@ -200,3 +200,84 @@ def test_server_request_errors(ctx):
     with pytest.raises(ba.FtlError):
         srv.download(cid)
     srv.close()
+
+
+def test_server_rejects_invalid_params_before_queueing(ctx):
+    """A creature with invalid parameters is refused at ftl_server_add even
+    when it would wait for memory, so it cannot block the creatures queued
+    behind it."""
+    shapes = [(600, 16, 31), (600, 16, 32)]
+    data = {seed: creature(S, N, seed) for S, N, seed in shapes}
+    per = 600 * 16 * 32 + 600 * 16 + 256
+    srv = ba.Server(ctx, hbm_budget_bytes=per + 1000)
+    a, res_a = srv.add(*data[31][:2], rest_roots=data[31][2], params=params())
+    assert res_a
+    bad = params()
+    bad.iterations = 0
+    with pytest.raises(ba.FtlError) as e:
+        srv.add(*data[32][:2], rest_roots=data[32][2], params=bad)  # would have to wait for memory
+    assert e.value.status == ba.FTL_ERR_INVALID_ARGUMENT
+    nan = params()
+    nan.damping = float("nan")
+    with pytest.raises(ba.FtlError):
+        srv.add(*data[32][:2], rest_roots=data[32][2], params=nan)
+    b, res_b = srv.add(*data[32][:2], rest_roots=data[32][2], params=params())
+    assert not res_b and srv.stats()["waiting"] == 1
+    t = srv.submit([b, b], requests(32, 2))
+    srv.remove(a)  # frees memory: the valid creature behind is admitted
+    assert srv.resident(b)
+    srv.wait(t[-1], 60000)
+    p, v = srv.download(b)
+    ref = plain_reference(ctx, *data[32], 2, 32)
+    assert_bitexact(p, ref[0], "admitted creature positions")
+    assert_bitexact(v, ref[1], "admitted creature velocities")
+    assert srv.stats()["failed_admissions"] == 0
+    srv.close()
+
+
+def test_server_quarantines_nonfinite_creatures(ctx):
+    """A creature whose state holds a NaN is quarantined after its first step:
+    its later requests are not stepped (FTL_ERR_NONFINITE), the other
+    creatures are unaffected, finite-only throughput leaves it out, and an
+    upload of clean state lifts the quarantine."""
+    shapes = [(400, 16, 41), (300, 32, 42)]
+    data = {seed: creature(S, N, seed) for S, N, seed in shapes}
+    srv = ba.Server(ctx)
+    ids = {seed: srv.add(*data[seed][:2], rest_roots=data[seed][2], params=params())[0] for _, _, seed in shapes}
+    pos, vel, roots = data[41]
+    bad_vel = vel.copy()
+    bad_vel[17, 5, 0] = np.nan
+    srv.upload(ids[41], pos, bad_vel)
+    t0 = srv.submit([ids[41], ids[42]], np.concatenate([requests(41, 1), requests(42, 1)]))
+    srv.wait(t0[0], 60000)  # the step itself ran
+    srv.wait(t0[1], 60000)
+    assert srv.status(ids[41]) and not srv.status(ids[42])
+    st = srv.stats()
+    assert st["quarantined"] == 1
+    assert st["cps_stepped"] - st["cps_finite"] == 400 * 16
+    # later requests of the quarantined creature complete without a step
+    p_before, v_before = srv.download(ids[41])
+    t1 = srv.submit([ids[41]], requests(41, 2)[1:2])
+    with pytest.raises(ba.FtlError) as e:
+        srv.wait(t1[0], 60000)
+    assert e.value.status == ba.FTL_ERR_NONFINITE
+    p_after, v_after = srv.download(ids[41])
+    assert np.array_equal(p_before.view(np.uint32), p_after.view(np.uint32))
+    assert srv.stats()["dropped"] == 1
+    # the neighbour steps on, bit-identical to its own reference
+    t2 = srv.submit([ids[42]], requests(42, 2)[1:2])
+    srv.wait(t2[0], 60000)
+    p, v = srv.download(ids[42])
+    ref = plain_reference(ctx, *data[42], 2, 42)
+    assert_bitexact(p, ref[0], "neighbour positions")
+    # clean state lifts the quarantine
+    srv.upload(ids[41], pos, vel)
+    assert not srv.status(ids[41])
+    t3 = srv.submit([ids[41]], requests(41, 1))
+    srv.wait(t3[0], 60000)
+    p, v = srv.download(ids[41])
+    ref = plain_reference(ctx, *data[41], 1, 41)
+    assert_bitexact(p, ref[0], "re-uploaded creature positions")
+    assert_bitexact(v, ref[1], "re-uploaded creature velocities")
+    assert srv.stats()["quarantined"] == 0
+    srv.close()
