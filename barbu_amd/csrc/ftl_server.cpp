@@ -97,7 +97,7 @@ struct ftl_server {
   uint64_t budget = 0, batch_cap = 0;
   int nslots = 2;
   // coalescing: a batch waits for this much work, or for its oldest request
-  // to be coalesce_us old, while another batch is on the device
+  // to be coalesce_us old
   uint64_t batch_target = 0;
   int coalesce_us = 0;
   std::mutex mu;
@@ -324,9 +324,9 @@ ftl_status launch_batch(ftl_server* s, Slot& sl) {
 // Requests of one batch: in arrival order, one per creature, idle resident
 // creatures only, up to the traffic cap (lock held).  Requests of a
 // quarantined creature complete at once with FTL_ERR_NONFINITE.  With
-// coalescing, while another batch is on the device, the selection is taken
-// only once it holds batch_target bytes or its oldest request is coalesce_us
-// old; otherwise nothing is taken and *deadline says when to look again.
+// coalescing, the selection is taken only once it holds batch_target bytes
+// or its oldest request is coalesce_us old; otherwise nothing is taken and
+// *deadline says when to look again.
 void select_batch(ftl_server* s, Slot& sl, Clock::time_point now, Clock::time_point* deadline) {
   sl.reqs.clear();
   uint64_t bytes = 0;
@@ -358,7 +358,10 @@ void select_batch(ftl_server* s, Slot& sl, Clock::time_point now, Clock::time_po
     ++it;
   }
   if (take.empty()) return;
-  if (s->coalesce_us > 0 && s->busy_slots > 0 && bytes < s->batch_target) {
+  // (Also with no batch on the device: a window that an idle device bypassed
+  // would let one slot launch a small batch whenever the device drains, and
+  // the batches would stay small under load.)
+  if (s->coalesce_us > 0 && bytes < s->batch_target) {
     const Clock::time_point due = oldest + std::chrono::microseconds(s->coalesce_us);
     if (now < due) {
       *deadline = std::min(*deadline, due);

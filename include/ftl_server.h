@@ -41,11 +41,11 @@ typedef struct ftl_server_config {
   uint64_t hbm_budget_bytes; /* device bytes resident creatures may hold; 0 = 80% of free device memory */
   uint64_t batch_cap_bytes;  /* HBM traffic per batch (64 B per control point); 0 = 8 GB */
   int max_inflight;          /* batches on the device at once, 1..4; 0 = 2 */
-  /* Coalescing window, microseconds (0 = none).  While another batch is on
-     the device, a new batch is launched only once its requests reach
+  /* Coalescing window, microseconds (0 = none: a batch launches as soon as a
+     slot is free).  A batch is launched only once its requests reach
      batch_target_bytes of traffic or its oldest request has waited
-     coalesce_us: under load, fewer and larger batches, traded against
-     latency.  With no batch on the device a batch always launches at once. */
+     coalesce_us: fewer and larger batches, traded against up to coalesce_us
+     of added latency. */
   int coalesce_us;
   uint64_t batch_target_bytes; /* 0 = the per-batch cap */
 } ftl_server_config;
