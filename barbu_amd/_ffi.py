@@ -195,6 +195,7 @@ SIGNATURES = {
     "ftl_groom_bind_skin": (st, [vp, vp, vp, vp]),
     "ftl_groom_root_frames": (st, [vp, vp, vp]),
     "ftl_batch_step": (st, [vp, vp, C.c_int, vp, vp]),
+    "ftl_batch_step_noalloc": (st, [vp, vp, C.c_int, vp, vp]),
     "ftl_batch_create": (st, [vp, vp, C.c_int, pp]),
     "ftl_batch_destroy": (None, [vp]),
     "ftl_batch_launch": (st, [vp, vp, vp]),

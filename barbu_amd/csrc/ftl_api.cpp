@@ -173,6 +173,8 @@ extern "C" ftl_status ftl_init(int device, ftl_ctx** out) {
   if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return ftl_hip_fail(e, "hipGetDeviceProperties");
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return ftl_fail(FTL_ERR_UNSUPPORTED, "libftl is built for gfx950; device %d is %s", device, prop.gcnArchName);
+  ftl_status st = ftl_noalloc_setup(device);
+  if (st != FTL_OK) return st;
   ftl_ctx* c = new ftl_ctx();
   c->device = device;
   c->ncu = prop.multiProcessorCount;
@@ -440,13 +442,17 @@ extern "C" ftl_status ftl_groom_set_lanes(ftl_groom* g, int lanes) {
 // Uploads the wind lattice when the wind parameters changed, in the device
 // layout of the kernels: per node (v.x, v.y, v.z, d.x), (d.y, d.z, 0, 0) with
 // d = v(x+1) - v(x) along the periodic x axis.
-static ftl_status ensure_wind(ftl_groom* g) {
+static ftl_status ensure_wind(ftl_groom* g, bool may_alloc) {
   const ftl_params& p = g->params;
   if (p.wind_amplitude == 0.0f) return FTL_OK;
   if (g->d_wind && g->wind_seed == p.wind_seed && g->wind_amp == p.wind_amplitude) return FTL_OK;
   const int G = FTL_WIND_GRID;
   const size_t cells = (size_t)G * G * G;
   const size_t bytes = cells * 2 * sizeof(float4);
+  if (!g->d_wind && !may_alloc)
+    return ftl_fail(FTL_ERR_INVALID_ARGUMENT,
+                    "the groom's wind lattice is not on the device yet (a step that may allocate, such as "
+                    "ftl_groom_step or ftl_batch_step, places it)");
   if (!g->d_wind) {
     ftl_status st = ftl_dev_alloc(g->ctx, (void**)&g->d_wind, bytes, "wind lattice");
     if (st != FTL_OK) return st;
@@ -474,9 +480,9 @@ static ftl_status ensure_wind(ftl_groom* g) {
   return FTL_OK;
 }
 
-ftl_status ftl_make_step(ftl_groom* g, float dt, FtlStep* A) {
+ftl_status ftl_make_step(ftl_groom* g, float dt, FtlStep* A, bool may_alloc) {
   if (!(dt > 0.0f) || !std::isfinite(dt)) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "dt must be finite and > 0");
-  ftl_status st = ensure_wind(g);
+  ftl_status st = ensure_wind(g, may_alloc);
   if (st != FTL_OK) return st;
   const ftl_params& p = g->params;
   std::memset(A, 0, sizeof *A);

@@ -188,6 +188,81 @@ extern "C" ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int
   return FTL_OK;
 }
 
+// ---------------------------------------------------------------------------
+// No-allocation batched step: the descriptors and the tile list live in a
+// file-scope device array of fixed size (one copy per GPU, held until the
+// process ends) instead of buffers the context allocates.  Host code never
+// uses the array's name as an address: its device address is looked up once
+// per device (ftl_noalloc_setup, from ftl_init) and kept in kNoalloc[dev].ptr.
+
+static __device__ __align__(256) unsigned char scratch[FTL_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct NoallocDevice {
+  std::mutex mu;            // one call at a time per device (host side)
+  bool ready = false;
+  void* ptr = nullptr;      // device address of `scratch` on this device
+  hipEvent_t done = nullptr;  // recorded after the last call's kernels
+  bool recorded = false;
+};
+NoallocDevice kNoalloc[kMaxDevices];
+}  // namespace
+
+ftl_status ftl_noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return ftl_fail(FTL_ERR_UNSUPPORTED, "device %d out of range", device);
+  NoallocDevice& d = kNoalloc[device];
+  std::lock_guard<std::mutex> lk(d.mu);
+  if (d.ready) return FTL_OK;
+  hipError_t e;
+  void* p = nullptr;
+  if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess || !p)
+    return ftl_hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+  if ((e = hipEventCreateWithFlags(&d.done, hipEventDisableTiming)) != hipSuccess)
+    return ftl_hip_fail(e, "hipEventCreate");
+  d.ptr = p;
+  d.ready = true;
+  return FTL_OK;
+}
+
+extern "C" ftl_status ftl_batch_step_noalloc(ftl_ctx* ctx, ftl_groom* const* grooms, int n, const float* dt,
+                                             void* stream) {
+  if (!ctx || (!grooms && n > 0) || (!dt && n > 0) || n < 0) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "bad arguments");
+  if (n == 0) return FTL_OK;
+  if (ctx->device < 0 || ctx->device >= kMaxDevices || !kNoalloc[ctx->device].ready)
+    return ftl_fail(FTL_ERR_NOT_INITIALIZED, "no-allocation scratch not set up on device %d", ctx->device);
+  std::vector<FtlTile> tiles;
+  std::vector<ClassRun> runs;
+  ftl_status st = ftl_plan_tiles(ctx, grooms, n, tiles, runs);
+  if (st != FTL_OK) return st;
+  const size_t sb = ((size_t)n * sizeof(FtlStep) + 255) & ~(size_t)255, tb = tiles.size() * sizeof(FtlTile);
+  if (sb + tb > FTL_NOALLOC_SCRATCH_BYTES)
+    return ftl_fail(FTL_ERR_INVALID_ARGUMENT,
+                    "batch too large for the no-allocation scratch: %d grooms (%zu B of descriptors) and %zu tiles "
+                    "(%zu B) exceed its %u bytes; use ftl_batch_step, or split the batch",
+                    n, sb, tiles.size(), tb, (unsigned)FTL_NOALLOC_SCRATCH_BYTES);
+  std::vector<FtlStep> steps(n);
+  for (int i = 0; i < n; ++i)
+    if ((st = ftl_make_step(grooms[i], dt[i], &steps[i], false)) != FTL_OK) return st;
+  hipError_t e;
+  if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
+  NoallocDevice& d = kNoalloc[ctx->device];
+  std::lock_guard<std::mutex> lk(d.mu);
+  // The previous call's kernels (on any stream) must have read the scratch
+  // before it is overwritten.
+  if (d.recorded && (e = hipEventSynchronize(d.done)) != hipSuccess) return ftl_hip_fail(e, "hipEventSynchronize");
+  unsigned char* base = (unsigned char*)d.ptr;
+  if ((e = hipMemcpy(base, steps.data(), (size_t)n * sizeof(FtlStep), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(base + sb, tiles.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
+    return ftl_hip_fail(e, "hipMemcpy to the no-allocation scratch");
+  if ((e = ftl_enqueue_runs(runs, (const FtlStep*)base, (const FtlTile*)(base + sb), (hipStream_t)stream, nullptr,
+                            nullptr, nullptr)) != hipSuccess)
+    return ftl_hip_fail(e, "batched step launch");
+  if ((e = hipEventRecord(d.done, (hipStream_t)stream)) != hipSuccess) return ftl_hip_fail(e, "hipEventRecord");
+  d.recorded = true;
+  return FTL_OK;
+}
+
 static void batch_release(ftl_batch* b) {
   ftl_ctx* ctx = b->ctx;
   (void)hipSetDevice(ctx->device);

@@ -93,7 +93,11 @@ ftl_status ftl_dev_alloc(ftl_ctx* ctx, void** p, size_t bytes, const char* what)
 ftl_status ftl_check_device_range(const void* p, size_t bytes, const char* what);
 void ftl_dev_free(ftl_ctx* ctx, void* p, size_t bytes);
 // Fills the kernel descriptor of one step of g.
-ftl_status ftl_make_step(ftl_groom* g, float dt, FtlStep* A);
+// may_alloc false: fail rather than allocate the groom's wind lattice.
+ftl_status ftl_make_step(ftl_groom* g, float dt, FtlStep* A, bool may_alloc = true);
+// Per-device set-up of the no-allocation batched step (its scratch address);
+// called by ftl_init with the device current.
+ftl_status ftl_noalloc_setup(int device);
 // Launch class for npoints: requested lanes (0 = tuned/default), at most mmax
 // points per lane.  -1 if none.
 // Launch class of a groom: lanes = the groom's setting (0 = tuned); wind: a

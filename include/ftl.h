@@ -274,6 +274,24 @@ ftl_status ftl_groom_root_frames(ftl_groom* g, float* roots, float* normals);
 ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int n, const float* dt,
                           void* stream);
 
+/* Device scratch of ftl_batch_step_noalloc: per GPU, held until the process
+   ends.  It holds the batch's descriptors (288 B per groom, rounded up to
+   256 B) and its tile list (16 B per 64-lane tile). */
+#define FTL_NOALLOC_SCRATCH_BYTES (4u << 20)
+/* ftl_batch_step without any allocation, for callers that cannot allocate
+   (code under HIP graph capture of their own, tight serving loops, harnesses
+   that forbid hipMalloc).  No counterpart in the reference.  The descriptors
+   and tiles go to a fixed device array instead of context-owned buffers;
+   batches that do not fit it (FTL_NOALLOC_SCRATCH_BYTES) are refused with
+   FTL_ERR_INVALID_ARGUMENT, and a groom whose wind lattice is not on the
+   device yet (it is placed by a first ftl_groom_step or ftl_batch_step) is
+   refused too.  Results are identical to ftl_batch_step.  Every call on one
+   GPU shares the array: a call waits (on the host) for the previous call's
+   kernels before it overwrites the array, and calls made concurrently from
+   several host threads on one GPU are serialised; the array is not meant to
+   be read by anything else. */
+ftl_status ftl_batch_step_noalloc(ftl_ctx* ctx, ftl_groom* const* grooms, int n, const float* dt, void* stream);
+
 /* A fixed set of grooms stepped as one frame, captured once into a HIP graph
    (parameter upload + one tile-list kernel per launch class) and replayed by
    ftl_batch_launch. */
