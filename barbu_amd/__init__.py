@@ -56,6 +56,13 @@ def make_params(**kw):
     return p
 
 
+def class_shape(cls):
+    """Shape of launch class cls: dict(points_per_lane, lanes, guarded, waves, chunk)."""
+    v = [C.c_int(0) for _ in range(5)]
+    check(lib().ftl_class_shape(int(cls), *[C.byref(x) for x in v]), "ftl_class_shape")
+    return dict(zip(("points_per_lane", "lanes", "guarded", "waves", "chunk"), (x.value for x in v)))
+
+
 def dualquat_from_matrix(m3x4):
     """The dual quaternion (8 floats: real xyzw, dual xyzw) of a rigid 3x4
     transform given as its three rows (ftl_dualquat_from_matrix; host only)."""
@@ -98,6 +105,10 @@ class Context:
         """Lanes per strand the library picks for npoints (plain steps with
         wind take their own table)."""
         return (lib().ftl_lanes_for_wind if wind else lib().ftl_lanes_for)(self._h, int(npoints))
+
+    def class_for(self, npoints, wind=False):
+        """The launch class (tile shape) the library picks for npoints."""
+        return int(lib().ftl_class_for_points(self._h, int(npoints), 1 if wind else 0))
 
     def close(self):
         if self._h:

@@ -207,6 +207,8 @@ SIGNATURES = {
     "ftl_diag_free": (st, [vp, vp, C.c_size_t]),
     "ftl_lanes_for": (C.c_int, [vp, C.c_int]),
     "ftl_lanes_for_wind": (C.c_int, [vp, C.c_int]),
+    "ftl_class_for_points": (C.c_int, [vp, C.c_int, C.c_int]),
+    "ftl_class_shape": (st, [C.c_int, ip, ip, ip, ip, ip]),
     "ftl_scn_sphere_groom": (st, [C.POINTER(ScnSphere), vp, vp, vp]),
     "ftl_scn_reference_groom": (st, [C.c_int, vp, vp, C.c_int, C.c_float, C.c_uint64, vp, vp]),
     "ftl_scn_skinned_sizes": (st, [C.POINTER(ScnSkinned), ip, ip]),

@@ -178,7 +178,10 @@ extern "C" ftl_status ftl_init(int device, ftl_ctx** out) {
   ftl_ctx* c = new ftl_ctx();
   c->device = device;
   c->ncu = prop.multiProcessorCount;
-  for (int i = 0; i <= FTL_MAX_POINTS; ++i) c->lanes[i] = c->lanes_wind[i] = 0;
+  for (int i = 0; i <= FTL_MAX_POINTS; ++i) {
+    c->lanes[i] = c->lanes_wind[i] = 0;
+    c->pick[i] = c->pick_wind[i] = -1;
+  }
   *out = c;
   return FTL_OK;
 }
@@ -219,6 +222,14 @@ static int default_lanes(int npoints) {
 }
 
 int ftl_class_for(ftl_ctx* ctx, int npoints, int lanes, int mmax, bool wind) {
+  // the tuned class of this point count (a base class or a tile-shape
+  // variant), when it has the requested lanes
+  if (ctx && npoints >= 0 && npoints <= FTL_MAX_POINTS) {
+    const int p = (wind && ctx->pick_wind[npoints] >= 0) ? ctx->pick_wind[npoints] : ctx->pick[npoints];
+    if (p >= 0 && p < FTL_NCLASSES && !kFtlClasses[p].guarded && kFtlClasses[p].m * kFtlClasses[p].g == npoints &&
+        kFtlClasses[p].m <= mmax && (lanes == 0 || lanes == kFtlClasses[p].g))
+      return p;
+  }
   int G = lanes;
   if (G == 0 && ctx && npoints >= 0 && npoints <= FTL_MAX_POINTS)
     G = (wind && ctx->lanes_wind[npoints]) ? ctx->lanes_wind[npoints] : ctx->lanes[npoints];
@@ -263,6 +274,22 @@ extern "C" int ftl_lanes_for(ftl_ctx* ctx, int npoints) {
 extern "C" int ftl_lanes_for_wind(ftl_ctx* ctx, int npoints) {
   const int c = ftl_class_for(ctx, npoints, 0, FTL_MAX_POINTS, true);
   return c < 0 ? 0 : kFtlClasses[c].g;
+}
+
+extern "C" int ftl_class_for_points(ftl_ctx* ctx, int npoints, int wind) {
+  return ftl_class_for(ctx, npoints, 0, FTL_MAX_POINTS, wind != 0);
+}
+
+extern "C" ftl_status ftl_class_shape(int cls, int* points_per_lane, int* lanes, int* guarded, int* waves,
+                                      int* chunk) {
+  if (cls < 0 || cls >= FTL_NCLASSES) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "no launch class %d", cls);
+  const FtlClassInfo& k = kFtlClasses[cls];
+  if (points_per_lane) *points_per_lane = k.m;
+  if (lanes) *lanes = k.g;
+  if (guarded) *guarded = k.guarded;
+  if (waves) *waves = k.waves;
+  if (chunk) *chunk = k.chunk < k.m ? k.chunk : k.m;
+  return FTL_OK;
 }
 
 // ---------------------------------------------------------------------------

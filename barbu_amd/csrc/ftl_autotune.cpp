@@ -1,8 +1,10 @@
-// Launch-shape autotuner: for each point count, times the exact launch
-// classes (lanes per strand 1, 2, 4) on a synthetic groom and keeps the
-// fastest.  The measured time per control point of every class is kept too:
-// batched frames share the device among classes in proportion to it.  Results do not depend on the choice (every class performs the
-// same IEEE operations per point); only the time does.
+// Launch-shape autotuner: for each point count, times every exact launch
+// class (lanes per strand 1, 2, 4 and the tile-shape variants of some: another
+// load/store chunk or occupancy target) on a synthetic groom and keeps the
+// fastest, without and with wind.  The measured time per control point of
+// every class is kept too: batched frames order their class kernels by it.
+// Results do not depend on the choice (every class performs the same IEEE
+// operations per point); only the time does.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -19,6 +21,35 @@
 #include "ftl_scenario.h"
 
 static const int kTunedPoints[] = {4, 8, 16, 32, 64};
+
+// A candidate for point count n: an exact class of n points whose lanes hold
+// at most 32 points (one lane per strand of 64 points does not fit).
+static bool candidate(int cls, int n) {
+  const FtlClassInfo& k = kFtlClasses[cls];
+  return !k.guarded && k.m * k.g == n && k.m <= 32;
+}
+
+// Reads `"name": { "n": v, ... }` into out[n] for the tuned point counts;
+// valid(n, v) filters the values.  Returns the number read.
+template <typename F>
+static int read_table(const std::string& s, const char* name, int* out, F valid) {
+  const size_t at = s.find(name);
+  if (at == std::string::npos) return 0;
+  const size_t end = s.find('}', at);
+  int found = 0;
+  char key[32];
+  for (int n : kTunedPoints) {
+    std::snprintf(key, sizeof key, "\"%d\": ", n);
+    const size_t k = s.find(key, at);
+    if (k == std::string::npos || k > end) continue;
+    const int v = std::atoi(s.c_str() + k + std::strlen(key));
+    if (valid(n, v)) {
+      out[n] = v;
+      ++found;
+    }
+  }
+  return found;
+}
 
 static bool load_cache(ftl_ctx* ctx, const char* path) {
   std::ifstream f(path);
@@ -41,48 +72,37 @@ static bool load_cache(ftl_ctx* ctx, const char* path) {
       if (v > 0.0f && v < 1e3f) ctx->cls_ns_cp[c] = v;
     }
   }
-  // "lanes": {...} and the optional "lanes_wind": {...}
-  auto read_table = [&](const char* name, int* out) {
-    const size_t at = s.find(name);
-    if (at == std::string::npos) return 0;
-    const size_t end = s.find('}', at);
-    int found = 0;
-    for (int n : kTunedPoints) {
-      std::snprintf(key, sizeof key, "\"%d\": ", n);
-      const size_t k = s.find(key, at);
-      if (k == std::string::npos || k > end) continue;
-      const int g = std::atoi(s.c_str() + k + std::strlen(key));
-      if (g == 1 || g == 2 || g == 4) {
-        out[n] = g;
-        ++found;
-      }
-    }
-    return found;
-  };
-  const int found = read_table("\"lanes\"", ctx->lanes);
-  read_table("\"lanes_wind\"", ctx->lanes_wind);
+  auto lanes_ok = [](int, int g) { return g == 1 || g == 2 || g == 4; };
+  auto class_ok = [](int n, int c) { return c >= 0 && c < FTL_NCLASSES && candidate(c, n); };
+  const int found = read_table(s, "\"lanes\"", ctx->lanes, lanes_ok);
+  read_table(s, "\"lanes_wind\"", ctx->lanes_wind, lanes_ok);
+  read_table(s, "\"class\"", ctx->pick, class_ok);
+  read_table(s, "\"class_wind\"", ctx->pick_wind, class_ok);
   return found > 0;
+}
+
+static void save_table(std::ofstream& f, const char* name, const int* t, bool last = false) {
+  f << "  \"" << name << "\": {";
+  bool first = true;
+  for (int n : kTunedPoints) {
+    if (t[n] <= 0 && std::strncmp(name, "class", 5) != 0) continue;
+    if (t[n] < 0) continue;
+    f << (first ? "" : ",") << "\n    \"" << n << "\": " << t[n];
+    first = false;
+  }
+  f << "\n  }" << (last ? "\n" : ",\n");
 }
 
 static void save_cache(ftl_ctx* ctx, const char* path, const std::vector<std::string>& notes) {
   std::ofstream f(path);
   if (!f) return;
-  f << "{\n  \"arch\": \"gfx950\",\n  \"ncu\": " << ctx->ncu << ",\n  \"lanes\": {";
+  f << "{\n  \"arch\": \"gfx950\",\n  \"ncu\": " << ctx->ncu << ",\n";
+  save_table(f, "lanes", ctx->lanes);
+  save_table(f, "lanes_wind", ctx->lanes_wind);
+  save_table(f, "class", ctx->pick);
+  save_table(f, "class_wind", ctx->pick_wind);
+  f << "  \"class_ns_per_cp\": {";
   bool first = true;
-  for (int n : kTunedPoints) {
-    if (!ctx->lanes[n]) continue;
-    f << (first ? "" : ",") << "\n    \"" << n << "\": " << ctx->lanes[n];
-    first = false;
-  }
-  f << "\n  },\n  \"lanes_wind\": {";
-  first = true;
-  for (int n : kTunedPoints) {
-    if (!ctx->lanes_wind[n]) continue;
-    f << (first ? "" : ",") << "\n    \"" << n << "\": " << ctx->lanes_wind[n];
-    first = false;
-  }
-  f << "\n  },\n  \"class_ns_per_cp\": {";
-  first = true;
   for (int c = 0; c < FTL_NCLASSES; ++c) {
     if (!(ctx->cls_ns_cp[c] > 0.0f)) continue;
     char v[48];
@@ -129,53 +149,54 @@ extern "C" ftl_status ftl_autotune(ftl_ctx* ctx, const char* cache_path, int for
     // with wind: plain steps with wind run the LDS-lattice form, and the
     // gathers change the balance between the load phase and the sweeps.
     for (int wind = 0; wind <= 1 && st == FTL_OK; ++wind) {
-    ftl_params p;
-    ftl_params_reference(&p);
-    p.force_scale = 1.0f;
-    p.length_scale = 1.0f;
-    p.damping = 0.9f;
-    p.wind_amplitude = wind ? 2.0f : 0.0f;
-    p.wind_frequency = 2.0f;
-    ftl_groom_set_params(g, &p);
-    int best_g = 0;
-    float best_t = 1e30f;
-    for (int G = 1; G <= 4; G *= 2) {
-      int cls = -1;
-      for (int c = 0; c < FTL_NCLASSES; ++c)
-        if (!kFtlClasses[c].guarded && kFtlClasses[c].g == G && kFtlClasses[c].m * G == N &&
-            kFtlClasses[c].m <= 32)
-          cls = c;
-      if (cls < 0) continue;
-      if ((st = ftl_groom_upload(g, pos.data(), vel.data())) != FTL_OK) break;
-      FtlStep A;
-      if ((st = ftl_make_step(g, 1.0f / 60.0f, &A)) != FTL_OK) break;
-      // Timed on a settled state: the first steps from the uploaded groom
-      // (strands still straight) collide far less than later ones.
-      for (int w = 0; w < 24; ++w) ftl_launch_plain(cls, A, nullptr);
-      const int reps = 10;
-      (void)hipEventRecord(t0, nullptr);
-      for (int r = 0; r < reps; ++r) ftl_launch_plain(cls, A, nullptr);
-      (void)hipEventRecord(t1, nullptr);
-      if ((e = hipEventSynchronize(t1)) != hipSuccess) {
-        st = ftl_hip_fail(e, "autotune timing");
-        break;
+      ftl_params p;
+      ftl_params_reference(&p);
+      p.force_scale = 1.0f;
+      p.length_scale = 1.0f;
+      p.damping = 0.9f;
+      p.wind_amplitude = wind ? 2.0f : 0.0f;
+      p.wind_frequency = 2.0f;
+      ftl_groom_set_params(g, &p);
+      int best = -1;
+      float best_t = 1e30f;
+      for (int cls = 0; cls < FTL_NCLASSES; ++cls) {
+        if (!candidate(cls, N)) continue;
+        if ((st = ftl_groom_upload(g, pos.data(), vel.data())) != FTL_OK) break;
+        FtlStep A;
+        if ((st = ftl_make_step(g, 1.0f / 60.0f, &A)) != FTL_OK) break;
+        // Timed on a settled state: the first steps from the uploaded groom
+        // (strands still straight) collide far less than later ones.
+        for (int w = 0; w < 24; ++w) ftl_launch_plain(cls, A, nullptr);
+        const int reps = 10;
+        (void)hipEventRecord(t0, nullptr);
+        for (int r = 0; r < reps; ++r) ftl_launch_plain(cls, A, nullptr);
+        (void)hipEventRecord(t1, nullptr);
+        if ((e = hipEventSynchronize(t1)) != hipSuccess) {
+          st = ftl_hip_fail(e, "autotune timing");
+          break;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        const float us = 1000.0f * ms / reps;
+        const FtlClassInfo& k = kFtlClasses[cls];
+        char note[160];
+        std::snprintf(note, sizeof note,
+                      "{\"npoints\": %d, \"class\": %d, \"lanes\": %d, \"waves\": %d, \"chunk\": %d, \"wind\": %d, "
+                      "\"us\": %.2f}",
+                      N, cls, k.g, k.waves, k.chunk < k.m ? k.chunk : k.m, wind, us);
+        notes.push_back(note);
+        // per-class cost, for ordering batched frames (which run without the
+        // LDS-lattice form): the timing without wind
+        if (!wind) ctx->cls_ns_cp[cls] = 1000.0f * us / ((float)S * (float)N);
+        if (us < best_t) {
+          best_t = us;
+          best = cls;
+        }
       }
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, t0, t1);
-      const float us = 1000.0f * ms / reps;
-      char note[112];
-      std::snprintf(note, sizeof note, "{\"npoints\": %d, \"lanes\": %d, \"wind\": %d, \"us\": %.2f}", N, G, wind,
-                    us);
-      notes.push_back(note);
-      // per-class cost, for ordering batched frames (which run without the
-      // LDS-lattice form): the timing without wind
-      if (!wind) ctx->cls_ns_cp[cls] = 1000.0f * us / ((float)S * (float)N);
-      if (us < best_t) {
-        best_t = us;
-        best_g = G;
+      if (st == FTL_OK && best >= 0) {
+        (wind ? ctx->lanes_wind : ctx->lanes)[N] = kFtlClasses[best].g;
+        (wind ? ctx->pick_wind : ctx->pick)[N] = best;
       }
-    }
-    if (st == FTL_OK && best_g) (wind ? ctx->lanes_wind : ctx->lanes)[N] = best_g;
     }
     ftl_groom_destroy(g);
     if (st != FTL_OK) break;

@@ -33,9 +33,22 @@
   X(19, 32, 2, true, 2, 8)     \
   X(20, 12, 2, true, 4, 4)     \
   X(21, 24, 2, true, 3, 8)     \
-  X(22, 16, 4, true, 3, 8)
+  X(22, 16, 4, true, 3, 8)     \
+  FTL_VARIANTS(X)
 
-#define FTL_NCLASSES 23
+// Tile-shape variants of exact classes: the same (M, G) with another load /
+// store chunk or occupancy target.  ftl_class_for never picks them on its
+// own; the autotuner times them beside the base classes and its tables
+// select them per point count.
+#define FTL_VARIANTS(X)        \
+  X(23, 16, 2, false, 2, 4)    \
+  X(24, 16, 4, false, 3, 4)    \
+  X(25, 8, 2, false, 4, 4)     \
+  X(26, 8, 4, false, 6, 8)     \
+  X(27, 32, 1, false, 2, 4)    \
+  X(28, 16, 1, false, 3, 4)
+
+#define FTL_NCLASSES 29
 
 struct FtlClassInfo {
   int m, g, guarded, waves, chunk;
@@ -47,10 +60,18 @@ static constexpr FtlClassInfo kFtlClasses[FTL_NCLASSES] = {
 #undef FTL_CI
 };
 
-// Load/store chunk of the class with these template parameters.
+// Load/store chunk of a tile of M points per lane.  A kernel translation unit
+// compiles one class (FTL_CLASS_ID), whose chunk it takes; elsewhere, the
+// first class with these parameters.
 constexpr int ftl_chunk_of(int m, int g, bool guarded) {
+#ifdef FTL_CLASS_ID
+  (void)g;
+  (void)guarded;
+  return kFtlClasses[FTL_CLASS_ID].chunk < m ? kFtlClasses[FTL_CLASS_ID].chunk : m;
+#else
   for (int i = 0; i < FTL_NCLASSES; ++i)
     if (kFtlClasses[i].m == m && kFtlClasses[i].g == g && (kFtlClasses[i].guarded != 0) == guarded)
       return kFtlClasses[i].chunk < m ? kFtlClasses[i].chunk : m;
   return m < 4 ? m : 4;
+#endif
 }

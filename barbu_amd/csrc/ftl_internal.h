@@ -16,6 +16,10 @@ struct ftl_ctx {
   int ncu = 0;
   int lanes[FTL_MAX_POINTS + 1];       // tuned lanes per strand by point count (0 = default rule)
   int lanes_wind[FTL_MAX_POINTS + 1];  // the same for plain launches with wind (0 = as lanes)
+  // tuned launch class (tile shape included) by point count, -1 = none: the
+  // class the tables pick among the base classes and their variants
+  int pick[FTL_MAX_POINTS + 1];
+  int pick_wind[FTL_MAX_POINTS + 1];
   float cls_ns_cp[FTL_NCLASSES] = {};  // measured ns per control point of each class (0 = not measured)
   std::mutex mu;
   uint64_t bytes = 0;  // device memory held by the context's objects

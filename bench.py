@@ -48,6 +48,12 @@ def context(ba):
     return ctx
 
 
+def set_tuning(path):
+    """Use another launch-shape table (a JSON written by ftl_autotune)."""
+    global TUNED
+    TUNED = os.path.abspath(path)
+
+
 def bench_single(ba, hipm, steps, warmup, S=262144, N=32, lanes=None):
     pos, vel, roots = ba.sphere_groom(S, N, radius=1.0, length=0.2, jitter=0.05, seed=2024)
     ctx = context(ba)
@@ -462,6 +468,7 @@ def main():
                     help="traffic at which a coalescing batch launches at once, MB")
     ap.add_argument("--skinned-steps", type=int, default=120)
     ap.add_argument("--batched-frames", type=int, default=60)
+    ap.add_argument("--tuning", default=None, help="launch-shape table to load instead of the shipped one")
     ap.add_argument("--lanes", type=int, default=0, choices=[0, 1, 2, 4],
                     help="lanes per strand of the single-groom workload (0: the tuned table)")
     args = ap.parse_args()
@@ -472,6 +479,9 @@ def main():
 
     import barbu_amd as ba
     from barbu_amd import _hip as hipm
+
+    if args.tuning:
+        set_tuning(args.tuning)
 
     log("single groom: 262144 x 32, %d warmup + %d timed steps" % (args.warmup, args.steps))
     r = bench_single(ba, hipm, args.steps, args.warmup, lanes=args.lanes or None)

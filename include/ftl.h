@@ -321,11 +321,13 @@ ftl_status ftl_diag_free(ftl_ctx* ctx, void* p, size_t bytes);
 
 /* -- launch-shape tuning ---------------------------------------------------- */
 
-/* Times the candidate lanes-per-strand for every point count on the current
-   device, without and with wind, and stores both choices in the context (used
-   when lanes = 0; plain steps with wind run a different kernel form, so they
-   take the second).  If cache_path is not NULL the tables are read from it
-   when present and written to it after tuning (JSON). */
+/* Times every exact launch class of each tuned point count (4, 8, 16, 32,
+   64) on the current device: the lanes-per-strand choices (1, 2, 4) and, for
+   some, tile-shape variants (another load/store chunk or occupancy target),
+   without and with wind, and stores the fastest class per point count in the
+   context (used when lanes = 0; plain steps with wind run a different kernel
+   form, so they take the second table).  If cache_path is not NULL the tables
+   are read from it when present and written to it after tuning (JSON). */
 ftl_status ftl_autotune(ftl_ctx* ctx, const char* cache_path, int force);
 /* The chosen lanes per strand for npoints (1, 2 or 4): batched and served
    steps, and plain steps without wind. */
@@ -333,6 +335,16 @@ int ftl_lanes_for(ftl_ctx* ctx, int npoints);
 /* The same for plain steps with wind.  No counterpart in the reference (its
    workgroup shape is fixed). */
 int ftl_lanes_for_wind(ftl_ctx* ctx, int npoints);
+/* The launch class the context picks for npoints (a plain step with wind
+   when wind != 0; batched and served steps use wind = 0), -1 if none.  A class
+   is a tile shape: points per lane, lanes per strand, the occupancy target the
+   registers are allocated for and the load/store chunk; the tuning tables may
+   pick a variant of a base class. */
+int ftl_class_for_points(ftl_ctx* ctx, int npoints, int wind);
+/* The shape of launch class cls (any pointer may be NULL): points per lane,
+   lanes per strand, guarded (1: ragged point counts), waves per SIMD the
+   register allocation targets, points per lane of a load/store chunk. */
+ftl_status ftl_class_shape(int cls, int* points_per_lane, int* lanes, int* guarded, int* waves, int* chunk);
 
 #ifdef __cplusplus
 }

@@ -138,3 +138,28 @@ def test_hair_mirror_uninitialised_update_is_a_noop():
     h.update(1 / 60)  # hair.cc:90-93: logs and returns
     h.deinit()
     assert not h.initialized()
+
+
+def test_launch_class_shapes():
+    """Every launch class reports its tile shape; the tile-shape variants are
+    exact classes with another chunk or occupancy target than their base."""
+    shapes = []
+    for cls in range(64):
+        try:
+            shapes.append(ba.class_shape(cls))
+        except ba.FtlError:
+            break
+    assert len(shapes) >= 29
+    for s in shapes:
+        assert s["lanes"] in (1, 2, 4) and 1 <= s["waves"] <= 8 and s["chunk"] in (2, 4, 8)
+        assert s["chunk"] <= s["points_per_lane"]
+    base = {}
+    for i, s in enumerate(shapes):
+        key = (s["points_per_lane"], s["lanes"], s["guarded"])
+        if key in base:
+            b = shapes[base[key]]
+            assert not s["guarded"] and (s["chunk"], s["waves"]) != (b["chunk"], b["waves"])
+        else:
+            base[key] = i
+    with pytest.raises(ba.FtlError):
+        ba.class_shape(len(shapes))

@@ -471,3 +471,64 @@ def test_batch_step_alternating_streams(ctx):
             p, v = g.download()
             assert_bitexact(p, s["pos"], "stream %s groom %d pos" % (name, i))
             assert_bitexact(v, s["vel"], "stream %s groom %d vel" % (name, i))
+
+
+def test_tuned_tile_shape_variants(tmp_path):
+    """A tuning table that picks tile-shape variants (another load/store chunk
+    or occupancy target) is loaded, the picked classes run plain and batched
+    steps, and the results stay bit-identical with the oracle."""
+    import json
+    import os
+    shipped = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "barbu_amd",
+                           "tuned_gfx950.json")
+    table = json.load(open(shipped))
+    variants = {}
+    for cls in range(64):
+        try:
+            s = ba.class_shape(cls)
+        except ba.FtlError:
+            break
+        n = s["points_per_lane"] * s["lanes"]
+        if not s["guarded"] and n in (8, 16, 32, 64) and s["points_per_lane"] <= 32:
+            variants.setdefault(n, []).append(cls)
+    pick = {n: max(c) for n, c in variants.items()}  # the last class of each count: a variant where one exists
+    table["class"] = {str(n): c for n, c in pick.items()}
+    table["class_wind"] = {str(n): c for n, c in pick.items()}
+    path = tmp_path / "tuned.json"
+    path.write_text(json.dumps(table))
+    c = ba.Context(0)
+    try:
+        c.autotune(str(path))
+        for n, cls in pick.items():
+            assert c.class_for(n) == cls and c.class_for(n, wind=True) == cls
+            assert c.lanes_for(n) == ba.class_shape(cls)["lanes"]
+        params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.2)
+        spheres = [(0, 0, 0, 1), (0.3, 1.1, 0.0, 0.35)]
+        for n in sorted(pick):
+            pos, vel, _ = ba.sphere_groom(700, n, radius=1.0, length=0.3, seed=90 + n)
+            run_compare(c, pos, vel, params, spheres, 2, label="variant class %d N=%d" % (pick[n], n))
+        # batched through the picked classes
+        grooms, states = [], []
+        for i, n in enumerate(sorted(pick)):
+            pos, vel, roots = ba.sphere_groom(900 + 37 * i, n, length=0.2, seed=120 + i)
+            g = ba.Groom(c, pos, vel, rest_roots=roots)
+            p = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0)
+            g.set_params(p)
+            sph = np.array([[0, 0, 0, 1.0]], np.float32)
+            g.set_colliders(sph)
+            g.set_root_mode(ba.FTL_ROOT_REST)
+            grooms.append(g)
+            states.append(dict(pos=pos.copy(), vel=vel.copy(), roots=roots, params=p, spheres=sph, N=n))
+        b = ba.Batch(c, grooms)
+        for k in range(2):
+            b.launch([DT] * len(grooms))
+            _oracle_frame(states, [DT] * len(grooms), [None] * len(grooms))
+        for i, (g, s) in enumerate(zip(grooms, states)):
+            p_, v_ = g.download()
+            assert_bitexact(p_, s["pos"], "variant batch groom %d pos" % i)
+            assert_bitexact(v_, s["vel"], "variant batch groom %d vel" % i)
+        b.close()
+        for g in grooms:
+            g.close()
+    finally:
+        c.close()
