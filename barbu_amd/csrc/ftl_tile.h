@@ -183,8 +183,62 @@ __device__ __forceinline__ void flush_chunk(float4* __restrict__ dst, int strand
   tile_sync();
 }
 
-// One sweep of DistanceConstraint over this lane's points (G == 1), fused with
-// the collisions of the points it has passed.  FINAL: last iteration, which
+// Lane-predicated collisions of one point: lanes with `on` false are left
+// untouched; `reflect` also reflects u (last iteration).
+__device__ __forceinline__ void collide_pred(const Spheres& S, float3& p, float3& u, bool on, bool reflect,
+                                             unsigned cmask) {
+#pragma unroll
+  for (int k = 0; k < FTL_NCOL; ++k) {
+    if (!has_collider(cmask, k)) continue;
+    const float3 c = f3(S.c[k][0], S.c[k][1], S.c[k][2]);
+    const float3 w = sub3(p, c);
+    const float dp = dot3(w, w);
+    if (on && dp < S.r2[k]) {
+      const float is = rsqrt_c(dp);
+      const float3 n = mul3(w, is);
+      p = fma3(S.r[k], n, c);
+      if (reflect) {
+        const float d2 = 2.0f * dot3(n, u);
+        u = fma3(-d2, n, u);
+      }
+    }
+  }
+}
+
+// Collisions of a sweep's points (not its last iteration), after all of the
+// sweep's constraints: a constraint reads its leader before the leader's
+// collision, so the collisions may follow the whole sweep.  Colliders outside,
+// points inside: each collider's centre and squared radius are copied to
+// vector registers once per sweep, since a vector instruction with a scalar
+// operand issues at half the rate of one with vector operands only when
+// several waves share a SIMD.  Each point still meets the colliders in order
+// with the same operations.  Point j takes part when on && j < jend && (j > 0
+// || on0).
+template <int M>
+__device__ __forceinline__ void collide_sweep(const Spheres& S, float3 (&p)[M], bool on, bool on0, int jend,
+                                              unsigned cmask) {
+#pragma unroll
+  for (int k = 0; k < FTL_NCOL; ++k) {
+    if (!has_collider(cmask, k)) continue;
+    float cx = S.c[k][0], cy = S.c[k][1], cz = S.c[k][2], r2 = S.r2[k];
+    asm volatile("" : "+v"(cx), "+v"(cy), "+v"(cz), "+v"(r2));
+    const float3 c = f3(cx, cy, cz);
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      if (j >= jend) continue;
+      const float3 w = sub3(p[j], c);
+      const float dp = dot3(w, w);
+      if (on && (j > 0 || on0) && dp < r2) {
+        const float is = rsqrt_c(dp);
+        const float3 n = mul3(w, is);
+        p[j] = fma3(S.r[k], n, c);
+      }
+    }
+  }
+}
+
+// One sweep of DistanceConstraint over this lane's points (G == 1), followed
+// by the collisions of its points.  FINAL: last iteration, which
 // also forms the velocities, stages them and flushes every completed chunk;
 // the lane's last point is left to the caller (its velocity is the raw
 // correction and its collision comes after the sweep).
@@ -230,51 +284,21 @@ __device__ __forceinline__ void sweep(const FtlStep& A, const VelArgs& V, const 
         clast = c;
         last_pre = pn;
       }
-    } else {
-      // Point j-1 is collided once, after the point it leads was swept.  The
-      // strand's last point is collided right after its own sweep (below);
-      // with GUARD the slots past it are not collided at all.
-      if (j >= 2 && (!GUARD || j - 1 < nl - 1)) {
-        float3 dummy;
-        collide<false>(S, p[j - 1], dummy, cm);
-      }
-      if (GUARD ? (j == nl - 1) : (j == M - 1)) {
-        last_pre = pn;
-        float3 dummy;
-        collide<false>(S, p[j], dummy, cm);
-      }
+    } else if (GUARD ? (j == nl - 1) : (j == M - 1)) {
+      last_pre = pn;
     }
   }
-}
-
-// Lane-predicated collisions of one point: lanes with `on` false are left
-// untouched; `reflect` also reflects u (last iteration).
-__device__ __forceinline__ void collide_pred(const Spheres& S, float3& p, float3& u, bool on, bool reflect,
-                                             unsigned cmask) {
-#pragma unroll
-  for (int k = 0; k < FTL_NCOL; ++k) {
-    if (!has_collider(cmask, k)) continue;
-    const float3 c = f3(S.c[k][0], S.c[k][1], S.c[k][2]);
-    const float3 w = sub3(p, c);
-    const float dp = dot3(w, w);
-    if (on && dp < S.r2[k]) {
-      const float is = rsqrt_c(dp);
-      const float3 n = mul3(w, is);
-      p = fma3(S.r[k], n, c);
-      if (reflect) {
-        const float d2 = 2.0f * dot3(n, u);
-        u = fma3(-d2, n, u);
-      }
-    }
-  }
+  // Collisions of points 1 .. last (the root never collides; with GUARD the
+  // slots past the last point are not collided at all).
+  if (!FINAL) collide_sweep<M>(S, p, true, false, GUARD ? nl : M, cm);
 }
 
 // One time step of the lane pipeline (G > 1): every lane runs the same code;
 // `active` lanes perform iteration t - g, `fin` lanes are in their last
 // iteration and also write their final velocities (lanes finish at different
 // time steps, so these go straight to memory; positions leave coalesced after
-// the pipeline drains).  State changes are selects, so no value is live across
-// the two sides of a divergent branch.  FINS: some lane may be in its last
+// the pipeline drains).  The 16-point forms update a point in place under the
+// lane mask; the smaller forms select.  FINS: some lane may be in its last
 // iteration (the last G time steps); the earlier steps skip the velocity work.
 // GUARD (strands of N = NS points, the last lane of a strand holding fewer
 // than M): tip is that lane's slot of the strand's last point (-1 on the other
@@ -292,14 +316,18 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, c
   constexpr int C = Tile<M, G, GUARD>::C;
   float sg = A.sigma;
   asm volatile("" : "+v"(sg));
-  const float dl = V.delta;
+  // The final steps' velocity constants in vector registers (vector-only
+  // operands issue at twice the rate of a scalar operand).
+  float dl = V.delta;
+  VelArgs Vv = V;
+  if constexpr (FINS) asm volatile("" : "+v"(dl), "+v"(Vv.inv_dt), "+v"(Vv.maxv2));
   if (!FINS) fin = false;
 #pragma unroll
   for (int j = 0; j < M; ++j) {
     const float3 q = p[j];
     const bool upd = active && (j != 0 || g != 0);  // lane 0's slot 0 is the root
     float3 pn;
-    if constexpr (kInPlacePipe<M> && !FINS) {
+    if constexpr (kInPlacePipe<M>) {
       // The point is updated in its own registers under the lane mask; its
       // leader is the previous point before that point's collision.
       if (upd) ftl_point(p[j], j == 0 ? lead : p[j - 1], sg * l.get(j));
@@ -331,13 +359,13 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, c
         // lane's last point (after the pipeline drains).
         if constexpr (kStagedVel<M, G>) {
           if (fin && real) {
-            const float3 v = out_velocity(V, pp, m, pm, u);
+            const float3 v = out_velocity(Vv, pp, m, pm, u);
             stage[slot<C>(lane, m % C)] = make_float4(v.x, v.y, v.z, 0.0f);
           }
           if (m % C == C - 1 && m / C < M / C - 1)
             flush_chunk<M, G, GUARD>(A.vel, strand0, A.nstrands, NS, m / C, stage, lane, gfin);
         } else if (fin && real && strand0 + lane / G < A.nstrands) {
-          const float3 v = out_velocity(V, pp, m, pm, u);
+          const float3 v = out_velocity(Vv, pp, m, pm, u);
           A.vel[(size_t)(strand0 + lane / G) * NS + g * M + m] = make_float4(v.x, v.y, v.z, 0.0f);
         }
       }
@@ -347,7 +375,7 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, c
           float3 pt = pn, ut = c;
           collide_pred(S, pt, ut, true, true, cm);
           p[j] = pt;
-          const float3 v = out_velocity(V, pp, j, pt, ut);
+          const float3 v = out_velocity(Vv, pp, j, pt, ut);
           if constexpr (kStagedVel<M, G>) {
             stage[slot<C>(lane, j % C)] = make_float4(v.x, v.y, v.z, 0.0f);
           } else if (strand0 + lane / G < A.nstrands) {
@@ -358,17 +386,18 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, c
         }
       }
       if (j == M - 1) clast = sel3(fin, c, clast);
-    } else if (j >= 1) {
-      const bool root = (j - 1 == 0) && (g == 0);
-      float3 dummy = f3(0.f, 0.f, 0.f);
-      collide_pred(S, p[j - 1], dummy, active && !root, false, cm);
     }
     if (j == M - 1) {
       last_pre = sel3(active, pn, last_pre);
-      float3 dummy = f3(0.f, 0.f, 0.f);
-      collide_pred(S, p[j], dummy, active && !fin, false, cm);  // deferred when final
+      if (FINS) {
+        float3 dummy = f3(0.f, 0.f, 0.f);
+        collide_pred(S, p[j], dummy, active && !fin, false, cm);  // deferred when final
+      }
     }
   }
+  // Before the final steps every point collides after the whole sweep (the
+  // root of lane 0 excepted).
+  if (!FINS) collide_sweep<M>(S, p, active, g != 0, M, cm);
 }
 
 // Shared memory of one tile: the load/store stage, and the rest lengths of
