@@ -70,12 +70,6 @@ struct Tile {
 template <int M, int G>
 constexpr bool kStagedVel = (M / Tile<M, G, false>::C) <= 2;
 
-// Pipelined forms with 16 points per lane update each point in place under
-// the lane mask (fewer selects and copies); the smaller forms keep selects,
-// which cost them fewer registers.
-template <int M>
-constexpr bool kInPlacePipe = (M == 16);
-
 // LDS slot of (owner lane r, point j of the chunk): XOR swizzle, conflict-free
 // for both the per-lane and the coalesced access order.
 template <int C>
@@ -297,8 +291,7 @@ __device__ __forceinline__ void sweep(const FtlStep& A, const VelArgs& V, const 
 // `active` lanes perform iteration t - g, `fin` lanes are in their last
 // iteration and also write their final velocities (lanes finish at different
 // time steps, so these go straight to memory; positions leave coalesced after
-// the pipeline drains).  The 16-point forms update a point in place under the
-// lane mask; the smaller forms select.  FINS: some lane may be in its last
+// the pipeline drains).  FINS: some lane may be in its last
 // iteration (the last G time steps); the earlier steps skip the velocity work.
 // GUARD (strands of N = NS points, the last lane of a strand holding fewer
 // than M): tip is that lane's slot of the strand's last point (-1 on the other
@@ -326,20 +319,11 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, c
   for (int j = 0; j < M; ++j) {
     const float3 q = p[j];
     const bool upd = active && (j != 0 || g != 0);  // lane 0's slot 0 is the root
-    float3 pn;
-    if constexpr (kInPlacePipe<M>) {
-      // The point is updated in its own registers under the lane mask; its
-      // leader is the previous point before that point's collision.
-      if (upd) ftl_point(p[j], j == 0 ? lead : p[j - 1], sg * l.get(j));
-      pn = p[j];
-    } else {
-      const float3 d = sub3(q, lead);
-      const float s = (sg * l.get(j)) * rsqrt_c(dot3(d, d));
-      pn = fma3(s, d, lead);
-      pn = sel3(upd, pn, q);
-      p[j] = pn;
-      lead = pn;
-    }
+    // The point is updated in its own registers under the lane mask (a select
+    // would be three vector instructions with a scalar mask operand); its
+    // leader is the previous point before that point's collision.
+    if (upd) ftl_point(p[j], j == 0 ? lead : p[j - 1], sg * l.get(j));
+    const float3 pn = p[j];
     if (FINS) {
       const float3 c = sub3(pn, q);
       if (j == 0) cfirst = sel3(fin, c, cfirst);
@@ -388,7 +372,9 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, c
       if (j == M - 1) clast = sel3(fin, c, clast);
     }
     if (j == M - 1) {
-      last_pre = sel3(active, pn, last_pre);
+      // An inactive lane's value is read only by the next lane while that lane
+      // is inactive too (fill and drain shift by one step per lane).
+      last_pre = pn;
       if (FINS) {
         float3 dummy = f3(0.f, 0.f, 0.f);
         collide_pred(S, p[j], dummy, active && !fin, false, cm);  // deferred when final
@@ -598,14 +584,15 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     for (int t = 0; t < K - 1; ++t) {
       const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
                            __shfl_up(last_pre.z, 1, G));
-      const float3 lead = sel3(g == 0, p[0], li);
+      // (lane 0 of a strand never reads its lead: its slot 0 is the root)
+      const float3 lead = li;
       sweep_pipe<M, G, false, GUARD>(A, V, S, p, l, lead, g, t >= g, false, -1, pp, stage, strand0, lane, u0,
                                      last_pre, cfirst, clast, cm, N, tip, &tbad);
     }
     for (int t = K - 1; t < K + G - 1; ++t) {
       const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
                            __shfl_up(last_pre.z, 1, G));
-      const float3 lead = sel3(g == 0, p[0], li);
+      const float3 lead = li;
       const int it = t - g;
       sweep_pipe<M, G, true, GUARD>(A, V, S, p, l, lead, g, it >= 0 && it < K, it == K - 1, t - (K - 1), pp,
                                     stage, strand0, lane, u0, last_pre, cfirst, clast, cm, N, tip, &tbad);

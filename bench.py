@@ -462,7 +462,7 @@ def main():
     ap.add_argument("--workload", default="all",
                     choices=["single", "batched", "served", "skinned", "expand", "marschner", "all"])
     ap.add_argument("--served-seconds", type=float, default=4.0)
-    ap.add_argument("--served-coalesce-us", type=int, default=1000,
+    ap.add_argument("--served-coalesce-us", type=int, default=2000,
                     help="coalescing window of the served loop (0: launch a batch as soon as a slot frees)")
     ap.add_argument("--served-target-mb", type=int, default=2048,
                     help="traffic at which a coalescing batch launches at once, MB")
