@@ -194,7 +194,7 @@ extern "C" ftl_status ftl_shutdown(ftl_ctx* ctx) {
     (void)hipEventDestroy(ctx->scratch_ev);
   }
   if (ctx->scratch_steps) ftl_dev_free(ctx, ctx->scratch_steps, ctx->scratch_steps_bytes);
-  if (ctx->scratch_tiles) ftl_dev_free(ctx, ctx->scratch_tiles, ctx->scratch_tiles_bytes);
+  if (ctx->scratch_spans) ftl_dev_free(ctx, ctx->scratch_spans, ctx->scratch_spans_bytes);
   if (ctx->live_objects != 0) {
     // Leaked handles keep their memory; the context itself is still released.
     ftl_fail(FTL_ERR_INVALID_ARGUMENT, "ftl_shutdown: %d objects still alive", ctx->live_objects);
@@ -215,6 +215,7 @@ extern "C" ftl_status ftl_ctx_memory(ftl_ctx* ctx, uint64_t* bytes) {
 // Default lanes per strand for a point count when no tuning is stored.
 static int default_lanes(int npoints) {
   switch (npoints) {
+    case 16: return 2;
     case 32: return 2;
     case 64: return 4;
     default: return 1;
@@ -275,6 +276,8 @@ extern "C" int ftl_lanes_for_wind(ftl_ctx* ctx, int npoints) {
   const int c = ftl_class_for(ctx, npoints, 0, FTL_MAX_POINTS, true);
   return c < 0 ? 0 : kFtlClasses[c].g;
 }
+
+int ftl_default_class(int npoints) { return ftl_class_for(nullptr, npoints, 0, FTL_MAX_POINTS); }
 
 extern "C" int ftl_class_for_points(ftl_ctx* ctx, int npoints, int wind) {
   return ftl_class_for(ctx, npoints, 0, FTL_MAX_POINTS, wind != 0);

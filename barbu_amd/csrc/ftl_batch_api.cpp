@@ -3,8 +3,10 @@
 // dispatches one workgroup per strand of its single groom (src/fx/hair.cc:112).
 //
 // A batch is planned once: each groom is cut into 64-lane tiles of its launch
-// class (points per lane x lanes per strand), and the tiles are grouped by
-// class.  Every class has its own tile-list kernel, compiled with the register
+// class (points per lane x lanes per strand), and the grooms are grouped by
+// class, each a span of consecutive tiles of its class's launch (one record
+// per groom: the kernel maps a tile to its groom by a binary search over the
+// spans).  Every class has its own tile-list kernel, compiled with the register
 // budget of that class (one kernel for all classes would carry the register
 // count of the heaviest one).  A frame is [upload the per-groom descriptors
 // from pinned memory -> one tile-list kernel per class present, as parallel
@@ -26,8 +28,8 @@ struct ftl_batch {
   ftl_ctx* ctx = nullptr;
   std::vector<ftl_groom*> grooms;
   std::vector<ClassRun> runs;
-  int ntiles = 0, grid = 0;
-  FtlTile* d_tiles = nullptr;
+  int ntiles = 0, grid = 0, nspans = 0;
+  FtlSpan* d_spans = nullptr;
   FtlStep* d_steps[2] = {nullptr, nullptr};
   FtlStep* h_steps[2] = {nullptr, nullptr};
   hipStream_t cap = nullptr;
@@ -55,11 +57,11 @@ static double class_ns_per_cp(const ftl_ctx* ctx, int cls, int K) {
 
 // Plans the tiles of a set of grooms, grouped by launch class (the class with
 // the most expected work first; within a class, grooms with more iterations
-// first).  Linear in the number of tiles (only the grooms are sorted): the
-// serving loop plans a batch per launch.
-ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::vector<FtlTile>& tiles,
+// first), as one span per groom.  Linear in the number of grooms: the serving
+// loop plans a batch per launch.
+ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::vector<FtlSpan>& spans,
                           std::vector<ClassRun>& runs) {
-  tiles.clear();
+  spans.clear();
   runs.clear();
   struct GroomPlan {
     int idx, cls, cost, ntiles;
@@ -100,19 +102,21 @@ ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::ve
     }
     return a.cost > b.cost;
   });
-  tiles.reserve(total_tiles);
+  spans.reserve(n);
   // One workgroup per tile: the classes' kernels run side by side and the
   // hardware dispatcher hands free wave slots to whichever has tiles left.
   for (size_t i = 0; i < plan.size();) {
     const int cls = plan[i].cls;
-    const int first = (int)tiles.size();
+    const int first = (int)spans.size();
+    int ntiles = 0;
     for (; i < plan.size() && plan[i].cls == cls; ++i) {
       const GroomPlan& gp = plan[i];
-      const int spt = FTL_WAVE / kFtlClasses[cls].g;
-      for (int t = 0; t < gp.ntiles; ++t) tiles.push_back(FtlTile{gp.idx, t * spt, cls, gp.cost});
+      if (gp.ntiles <= 0) continue;  // no strands: nothing to step
+      spans.push_back(FtlSpan{gp.idx, ntiles, cls, gp.cost});
+      ntiles += gp.ntiles;
     }
-    const int count = (int)tiles.size() - first;
-    runs.push_back(ClassRun{cls, first, count, count});
+    const int count = (int)spans.size() - first;
+    if (count > 0) runs.push_back(ClassRun{cls, first, count, ntiles, ntiles});
   }
   if (runs.size() > 64) return ftl_fail(FTL_ERR_UNSUPPORTED, "too many launch classes");
   return FTL_OK;
@@ -121,7 +125,7 @@ ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::ve
 // Enqueues one tile-list kernel per class on s.  With side streams (graph
 // capture, serving slots) the classes after the first are forked onto them
 // and joined back, so they run side by side (parallel graph branches).
-hipError_t ftl_enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_steps, const FtlTile* d_tiles,
+hipError_t ftl_enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_steps, const FtlSpan* d_spans,
                             hipStream_t s, const std::vector<hipStream_t>* side, const std::vector<hipEvent_t>* side_ev,
                             hipEvent_t fork) {
   hipError_t e;
@@ -131,7 +135,8 @@ hipError_t ftl_enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_
     hipStream_t rs = (par && r > 0) ? (*side)[r - 1] : s;
     if (par && r > 0 && (e = hipStreamWaitEvent(rs, fork, 0)) != hipSuccess) return e;
     const ClassRun& cr = runs[r];
-    if ((e = ftl_launch_tiles(cr.cls, d_steps, d_tiles + cr.first, cr.count, cr.grid, rs)) != hipSuccess) return e;
+    if ((e = ftl_launch_tiles(cr.cls, d_steps, d_spans + cr.first, cr.count, cr.ntiles, cr.grid, rs)) != hipSuccess)
+      return e;
     if (par && r > 0) {
       if ((e = hipEventRecord((*side_ev)[r - 1], rs)) != hipSuccess) return e;
       if ((e = hipStreamWaitEvent(s, (*side_ev)[r - 1], 0)) != hipSuccess) return e;
@@ -143,16 +148,17 @@ hipError_t ftl_enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_
 extern "C" ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int n, const float* dt, void* stream) {
   if (!ctx || (!grooms && n > 0) || (!dt && n > 0) || n < 0) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "bad arguments");
   if (n == 0) return FTL_OK;
-  std::vector<FtlTile> tiles;
+  std::vector<FtlSpan> spans;
   std::vector<ClassRun> runs;
-  ftl_status st = ftl_plan_tiles(ctx, grooms, n, tiles, runs);
+  ftl_status st = ftl_plan_tiles(ctx, grooms, n, spans, runs);
   if (st != FTL_OK) return st;
   std::vector<FtlStep> steps(n);
   for (int i = 0; i < n; ++i)
     if ((st = ftl_make_step(grooms[i], dt[i], &steps[i])) != FTL_OK) return st;
+  if (spans.empty()) return FTL_OK;
   hipError_t e;
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
-  const size_t sb = steps.size() * sizeof(FtlStep), tb = tiles.size() * sizeof(FtlTile);
+  const size_t sb = steps.size() * sizeof(FtlStep), tb = spans.size() * sizeof(FtlSpan);
   // The scratch buffers may still be read by the previous batch step, which
   // may have run on another stream: wait for its kernels before the
   // descriptors are replaced (or the buffers freed to grow).
@@ -168,18 +174,18 @@ extern "C" ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int
     if ((st = ftl_dev_alloc(ctx, &ctx->scratch_steps, sb, "batch descriptors")) != FTL_OK) return st;
     ctx->scratch_steps_bytes = sb;
   }
-  if (tb > ctx->scratch_tiles_bytes) {
-    ftl_dev_free(ctx, ctx->scratch_tiles, ctx->scratch_tiles_bytes);
-    ctx->scratch_tiles = nullptr;
-    ctx->scratch_tiles_bytes = 0;
-    if ((st = ftl_dev_alloc(ctx, &ctx->scratch_tiles, tb, "batch tiles")) != FTL_OK) return st;
-    ctx->scratch_tiles_bytes = tb;
+  if (tb > ctx->scratch_spans_bytes) {
+    ftl_dev_free(ctx, ctx->scratch_spans, ctx->scratch_spans_bytes);
+    ctx->scratch_spans = nullptr;
+    ctx->scratch_spans_bytes = 0;
+    if ((st = ftl_dev_alloc(ctx, &ctx->scratch_spans, tb, "batch spans")) != FTL_OK) return st;
+    ctx->scratch_spans_bytes = tb;
   }
   if ((e = hipMemcpy(ctx->scratch_steps, steps.data(), sb, hipMemcpyHostToDevice)) != hipSuccess)
     return ftl_hip_fail(e, "hipMemcpy batch descriptors");
-  if ((e = hipMemcpy(ctx->scratch_tiles, tiles.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
-    return ftl_hip_fail(e, "hipMemcpy batch tiles");
-  if ((e = ftl_enqueue_runs(runs, (const FtlStep*)ctx->scratch_steps, (const FtlTile*)ctx->scratch_tiles,
+  if ((e = hipMemcpy(ctx->scratch_spans, spans.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
+    return ftl_hip_fail(e, "hipMemcpy batch spans");
+  if ((e = ftl_enqueue_runs(runs, (const FtlStep*)ctx->scratch_steps, (const FtlSpan*)ctx->scratch_spans,
                             (hipStream_t)stream, nullptr, nullptr, nullptr)) != hipSuccess)
     return ftl_hip_fail(e, "batched step launch");
   if ((e = hipEventRecord(ctx->scratch_ev, (hipStream_t)stream)) != hipSuccess)
@@ -189,7 +195,7 @@ extern "C" ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int
 }
 
 // ---------------------------------------------------------------------------
-// No-allocation batched step: the descriptors and the tile list live in a
+// No-allocation batched step: the descriptors and the span list live in a
 // file-scope device array of fixed size (one copy per GPU, held until the
 // process ends) instead of buffers the context allocates.  Host code never
 // uses the array's name as an address: its device address is looked up once
@@ -231,16 +237,18 @@ extern "C" ftl_status ftl_batch_step_noalloc(ftl_ctx* ctx, ftl_groom* const* gro
   if (n == 0) return FTL_OK;
   if (ctx->device < 0 || ctx->device >= kMaxDevices || !kNoalloc[ctx->device].ready)
     return ftl_fail(FTL_ERR_NOT_INITIALIZED, "no-allocation scratch not set up on device %d", ctx->device);
-  std::vector<FtlTile> tiles;
-  std::vector<ClassRun> runs;
-  ftl_status st = ftl_plan_tiles(ctx, grooms, n, tiles, runs);
-  if (st != FTL_OK) return st;
-  const size_t sb = ((size_t)n * sizeof(FtlStep) + 255) & ~(size_t)255, tb = tiles.size() * sizeof(FtlTile);
-  if (sb + tb > FTL_NOALLOC_SCRATCH_BYTES)
+  // At most one span per groom: the size is checked before any planning.
+  const size_t sb = ((size_t)n * sizeof(FtlStep) + 255) & ~(size_t)255;
+  if (sb + (size_t)n * sizeof(FtlSpan) > FTL_NOALLOC_SCRATCH_BYTES)
     return ftl_fail(FTL_ERR_INVALID_ARGUMENT,
-                    "batch too large for the no-allocation scratch: %d grooms (%zu B of descriptors) and %zu tiles "
-                    "(%zu B) exceed its %u bytes; use ftl_batch_step, or split the batch",
-                    n, sb, tiles.size(), tb, (unsigned)FTL_NOALLOC_SCRATCH_BYTES);
+                    "batch too large for the no-allocation scratch: %d grooms (%zu B of descriptors, %zu B of "
+                    "spans) exceed its %u bytes; use ftl_batch_step, or split the batch",
+                    n, sb, (size_t)n * sizeof(FtlSpan), (unsigned)FTL_NOALLOC_SCRATCH_BYTES);
+  std::vector<FtlSpan> spans;
+  std::vector<ClassRun> runs;
+  ftl_status st = ftl_plan_tiles(ctx, grooms, n, spans, runs);
+  if (st != FTL_OK) return st;
+  const size_t tb = spans.size() * sizeof(FtlSpan);
   std::vector<FtlStep> steps(n);
   for (int i = 0; i < n; ++i)
     if ((st = ftl_make_step(grooms[i], dt[i], &steps[i], false)) != FTL_OK) return st;
@@ -253,9 +261,9 @@ extern "C" ftl_status ftl_batch_step_noalloc(ftl_ctx* ctx, ftl_groom* const* gro
   if (d.recorded && (e = hipEventSynchronize(d.done)) != hipSuccess) return ftl_hip_fail(e, "hipEventSynchronize");
   unsigned char* base = (unsigned char*)d.ptr;
   if ((e = hipMemcpy(base, steps.data(), (size_t)n * sizeof(FtlStep), hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(base + sb, tiles.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
+      (tb > 0 && (e = hipMemcpy(base + sb, spans.data(), tb, hipMemcpyHostToDevice)) != hipSuccess))
     return ftl_hip_fail(e, "hipMemcpy to the no-allocation scratch");
-  if ((e = ftl_enqueue_runs(runs, (const FtlStep*)base, (const FtlTile*)(base + sb), (hipStream_t)stream, nullptr,
+  if ((e = ftl_enqueue_runs(runs, (const FtlStep*)base, (const FtlSpan*)(base + sb), (hipStream_t)stream, nullptr,
                             nullptr, nullptr)) != hipSuccess)
     return ftl_hip_fail(e, "batched step launch");
   if ((e = hipEventRecord(d.done, (hipStream_t)stream)) != hipSuccess) return ftl_hip_fail(e, "hipEventRecord");
@@ -274,7 +282,7 @@ static void batch_release(ftl_batch* b) {
     if (b->h_steps[k]) (void)hipHostFree(b->h_steps[k]);
     ftl_dev_free(ctx, b->d_steps[k], b->grooms.size() * sizeof(FtlStep));
   }
-  ftl_dev_free(ctx, b->d_tiles, (size_t)b->ntiles * sizeof(FtlTile));
+  ftl_dev_free(ctx, b->d_spans, (size_t)b->nspans * sizeof(FtlSpan));
   for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
   for (hipEvent_t ev : b->side_ev) (void)hipEventDestroy(ev);
   if (b->fork) (void)hipEventDestroy(b->fork);
@@ -284,27 +292,32 @@ static void batch_release(ftl_batch* b) {
 extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, int n, ftl_batch** out) {
   if (!ctx || !grooms || !out || n <= 0) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "bad arguments");
   *out = nullptr;
-  std::vector<FtlTile> tiles;
+  std::vector<FtlSpan> spans;
   std::vector<ClassRun> runs;
-  ftl_status st = ftl_plan_tiles(ctx, grooms, n, tiles, runs);
+  ftl_status st = ftl_plan_tiles(ctx, grooms, n, spans, runs);
   if (st != FTL_OK) return st;
   hipError_t e;
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
   ftl_batch* b = new ftl_batch();
   b->ctx = ctx;
   b->grooms.assign(grooms, grooms + n);
-  b->ntiles = (int)tiles.size();
+  b->nspans = (int)spans.size();
   b->runs = runs;
-  for (const ClassRun& r : runs) b->grid += r.grid;
-  const size_t sb = (size_t)n * sizeof(FtlStep), tb = tiles.size() * sizeof(FtlTile);
+  for (const ClassRun& r : runs) {
+    b->ntiles += r.ntiles;
+    b->grid += r.grid;
+  }
+  const size_t sb = (size_t)n * sizeof(FtlStep), tb = spans.size() * sizeof(FtlSpan);
   auto bail = [&](ftl_status s) {
     batch_release(b);
     delete b;
     return s;
   };
-  if ((st = ftl_dev_alloc(ctx, (void**)&b->d_tiles, tb, "batch tiles")) != FTL_OK) return bail(st);
-  if ((e = hipMemcpy(b->d_tiles, tiles.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
-    return bail(ftl_hip_fail(e, "hipMemcpy batch tiles"));
+  if (tb > 0) {
+    if ((st = ftl_dev_alloc(ctx, (void**)&b->d_spans, tb, "batch spans")) != FTL_OK) return bail(st);
+    if ((e = hipMemcpy(b->d_spans, spans.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
+      return bail(ftl_hip_fail(e, "hipMemcpy batch spans"));
+  }
   if ((e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking)) != hipSuccess)
     return bail(ftl_hip_fail(e, "hipStreamCreate"));
   if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess)
@@ -335,7 +348,7 @@ extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, i
     hipError_t e1 = FTL_KERNEL_UPLOAD ? ftl_launch_upload(b->h_steps[k], b->d_steps[k], sb, b->cap)
                                       : hipMemcpyAsync(b->d_steps[k], b->h_steps[k], sb, hipMemcpyHostToDevice, b->cap);
     hipError_t e2 = e1 == hipSuccess
-                        ? ftl_enqueue_runs(b->runs, b->d_steps[k], b->d_tiles, b->cap, &b->side, &b->side_ev, b->fork)
+                        ? ftl_enqueue_runs(b->runs, b->d_steps[k], b->d_spans, b->cap, &b->side, &b->side_ev, b->fork)
                         : hipSuccess;
     hipError_t e3 = hipStreamEndCapture(b->cap, &b->graph[k]);
     if (e1 != hipSuccess) return bail(ftl_hip_fail(e1, "capture: descriptor upload"));

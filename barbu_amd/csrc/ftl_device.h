@@ -39,12 +39,15 @@ struct FtlStep {
   float wind_off[4];            // lattice offset of this step (w unused)
 };
 
-// Work unit of a batched launch: 64 lanes of one groom.
-struct FtlTile {
-  int groom;    // index into the FtlStep table
-  int strand0;  // first strand of the tile
-  int cls;      // launch class (points per lane, lanes per strand)
-  int cost;     // relative cost, for ordering
+// One groom's share of a class's batched launch: tiles tile0, tile0 + 1, ...
+// of the class's kernel step the groom's strands, 64 / G strands a tile.  A
+// launch's spans are in tile order, and each workgroup finds its own by a
+// binary search, so a batch uploads one record per groom, not per tile.
+struct FtlSpan {
+  int groom;  // index into the FtlStep table
+  int tile0;  // first tile of the groom in its class's launch
+  int cls;    // launch class (points per lane, lanes per strand)
+  int cost;   // relative cost of a tile, for ordering
 };
 
 namespace ftl_dev {

@@ -31,8 +31,8 @@ struct ftl_ctx {
   bool scratch_ev_recorded = false;
   void* scratch_steps = nullptr;
   size_t scratch_steps_bytes = 0;
-  void* scratch_tiles = nullptr;
-  size_t scratch_tiles_bytes = 0;
+  void* scratch_spans = nullptr;
+  size_t scratch_spans_bytes = 0;
 };
 
 struct ftl_skin {
@@ -107,19 +107,24 @@ ftl_status ftl_noalloc_setup(int device);
 // Launch class of a groom: lanes = the groom's setting (0 = tuned); wind: a
 // plain launch with wind (it runs the LDS-lattice form, tuned separately).
 int ftl_class_for(ftl_ctx* ctx, int npoints, int lanes, int mmax, bool wind = false);
+// The class of npoints without tuning tables (the built-in lanes rule).
+int ftl_default_class(int npoints);
 
 // Batched steps (ftl_batch_api.cpp), shared by the batch objects and the
 // serving loop.
 // One persistent launch: the tiles [first, first + count) of one class.
+// One class's launch in a batch: spans [first, first + count) of the batch's
+// span list, ntiles tiles in all, stepped by grid workgroups.
 struct ClassRun {
-  int cls, first, count, grid;
+  int cls, first, count, ntiles, grid;
 };
-// Cuts the grooms into tiles grouped by class and sizes the class grids.
-ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::vector<FtlTile>& tiles,
+// Groups the grooms by class (one span per groom, in tile order within its
+// class) and sizes the class grids.
+ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::vector<FtlSpan>& spans,
                           std::vector<ClassRun>& runs);
 // Enqueues one tile-list kernel per class on s; with side streams (and their
 // events) the classes after the first run on them, forked from and joined
 // back to s.
-hipError_t ftl_enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_steps, const FtlTile* d_tiles,
+hipError_t ftl_enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_steps, const FtlSpan* d_spans,
                             hipStream_t s, const std::vector<hipStream_t>* side, const std::vector<hipEvent_t>* side_ev,
                             hipEvent_t fork);

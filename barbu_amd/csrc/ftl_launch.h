@@ -14,9 +14,10 @@ int ftl_class_guarded(int cls);
 // start / stop: optional events bracketing the kernel itself (timing).
 hipError_t ftl_launch_plain(int cls, const FtlStep& A, hipStream_t stream, hipEvent_t start = nullptr,
                             hipEvent_t stop = nullptr);
-// Tile-list kernel of one class over tiles[0..ntiles) (all of that class):
-// workgroup b steps tiles b, b + grid, ...
-hipError_t ftl_launch_tiles(int cls, const FtlStep* grooms, const FtlTile* tiles, int ntiles, int grid,
+// Tile-list kernel of one class over the ntiles tiles of spans[0..nspans)
+// (every groom of that class, in tile order): workgroup b steps tiles b,
+// b + grid, ...
+hipError_t ftl_launch_tiles(int cls, const FtlStep* grooms, const FtlSpan* spans, int nspans, int ntiles, int grid,
                             hipStream_t stream);
 // Resident tile-list workgroups per CU for a class.
 int ftl_tiles_blocks_per_cu(int cls);

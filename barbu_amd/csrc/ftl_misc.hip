@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void ftl_upload_kernel(const uint4* __restrict
 
 #define FTL_DECL(id, M, G, GU, W, CH)                                                                    \
   hipError_t ftl_launch_plain_##id(const FtlStep& A, hipStream_t stream, hipEvent_t, hipEvent_t);      \
-  hipError_t ftl_launch_tiles_##id(const FtlStep*, const FtlTile*, int, int, hipStream_t);          \
+  hipError_t ftl_launch_tiles_##id(const FtlStep*, const FtlSpan*, int, int, int, hipStream_t);     \
   int ftl_tiles_occ_##id();
 FTL_CLASSES(FTL_DECL)
 #undef FTL_DECL
@@ -197,12 +197,12 @@ hipError_t ftl_launch_plain(int cls, const FtlStep& A, hipStream_t stream, hipEv
   return hipErrorInvalidValue;
 }
 
-hipError_t ftl_launch_tiles(int cls, const FtlStep* grooms, const FtlTile* tiles, int ntiles, int grid,
+hipError_t ftl_launch_tiles(int cls, const FtlStep* grooms, const FtlSpan* spans, int nspans, int ntiles, int grid,
                             hipStream_t stream) {
   switch (cls) {
 #define FTL_LB(id, M, G, GU, W, CH) \
   case id:                   \
-    return ftl_launch_tiles_##id(grooms, tiles, ntiles, grid, stream);
+    return ftl_launch_tiles_##id(grooms, spans, nspans, ntiles, grid, stream);
     FTL_CLASSES(FTL_LB)
 #undef FTL_LB
   }
@@ -230,7 +230,7 @@ hipError_t ftl_launch_root_frames(int nstrands, const int4* bind, const int4* tr
   return hipGetLastError();
 }
 
-static_assert(sizeof(FtlStep) % 16 == 0 && sizeof(FtlTile) % 16 == 0, "descriptors upload in 16-byte units");
+static_assert(sizeof(FtlStep) % 16 == 0 && sizeof(FtlSpan) % 16 == 0, "descriptors upload in 16-byte units");
 
 hipError_t ftl_launch_upload(const void* host_src, void* dst, size_t bytes, hipStream_t stream) {
   if (bytes % 16 != 0) return hipErrorInvalidValue;

@@ -125,8 +125,10 @@ hipError_t FTL_PASTE(ftl_launch_plain_, FTL_CLASS_ID)(const FtlStep& A, hipStrea
 
 // ---------------------------------------------------------------------------
 // Tile-list form of the same class for batched frames: workgroup b steps the
-// tiles b, b + grid, b + 2 grid, ... of a list of 64-lane tiles drawn from any
-// grooms of this class.  With grid = number of tiles (the batched default)
+// tiles b, b + grid, b + 2 grid, ... of the 64-lane tiles of any grooms of
+// this class, each groom a span of consecutive tiles; a workgroup finds its
+// tile's groom by a binary search over the spans' first tiles (scalar loads,
+// a few per tile).  With grid = number of tiles (the batched default)
 // each workgroup takes one tile and the hardware dispatcher balances the
 // load; a smaller grid makes the workgroups persistent.  (A shared atomic
 // work counter was measured 2-5x slower: every claim contends on one address
@@ -135,22 +137,32 @@ hipError_t FTL_PASTE(ftl_launch_plain_, FTL_CLASS_ID)(const FtlStep& A, hipStrea
 namespace {
 
 __global__ __launch_bounds__(FTL_WAVE) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
-void ftl_tiles_kernel(const FtlStep* __restrict__ grooms, const FtlTile* __restrict__ tiles, int ntiles) {
+void ftl_tiles_kernel(const FtlStep* __restrict__ grooms, const FtlSpan* __restrict__ spans, int nspans,
+                      int ntiles) {
   __shared__ ftl_tile_impl::TileLds<CL::M, CL::G, CL::GU> sh;
+  constexpr int spt = FTL_WAVE / CL::G;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const FtlTile td = tiles[t];
-    const int gi = __builtin_amdgcn_readfirstlane(td.groom);
-    const int s0 = __builtin_amdgcn_readfirstlane(td.strand0);
+    // the last span whose first tile is at or before t (spans[0].tile0 == 0)
+    int lo = 0, hi = nspans - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (__builtin_amdgcn_readfirstlane(spans[mid].tile0) <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int gi = __builtin_amdgcn_readfirstlane(spans[lo].groom);
+    const int s0 = (t - __builtin_amdgcn_readfirstlane(spans[lo].tile0)) * spt;
     ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead>(grooms[gi], s0, sh);
   }
 }
 
 }  // namespace
 
-hipError_t FTL_PASTE(ftl_launch_tiles_, FTL_CLASS_ID)(const FtlStep* grooms, const FtlTile* tiles, int ntiles,
-                                                      int grid, hipStream_t stream) {
-  if (ntiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(ftl_tiles_kernel, dim3(grid), dim3(FTL_WAVE), 0, stream, grooms, tiles, ntiles);
+hipError_t FTL_PASTE(ftl_launch_tiles_, FTL_CLASS_ID)(const FtlStep* grooms, const FtlSpan* spans, int nspans,
+                                                      int ntiles, int grid, hipStream_t stream) {
+  if (ntiles <= 0 || nspans <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ftl_tiles_kernel, dim3(grid), dim3(FTL_WAVE), 0, stream, grooms, spans, nspans, ntiles);
   return hipGetLastError();
 }
 

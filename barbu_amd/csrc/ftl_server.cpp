@@ -73,9 +73,9 @@ struct Slot {
   FtlStep* h_steps = nullptr;
   FtlStep* d_steps = nullptr;
   size_t steps_cap = 0;
-  FtlTile* h_tiles = nullptr;
-  FtlTile* d_tiles = nullptr;
-  size_t tiles_cap = 0;
+  FtlSpan* h_spans = nullptr;
+  FtlSpan* d_spans = nullptr;
+  size_t spans_cap = 0;
   // Non-finite flag of each request of the batch: pinned, coherent host
   // memory the step kernels write directly (the groom's status word is
   // replaced by flags[i] in the batch's descriptors).
@@ -207,7 +207,7 @@ void admit_waiting(ftl_server* s) {
 #endif
 constexpr unsigned kHostUpload = FTL_KERNEL_UPLOAD ? hipHostMallocCoherent : hipHostMallocDefault;
 
-hipError_t ensure_slot(ftl_server* s, Slot& sl, size_t nsteps, size_t ntiles, size_t nruns) {
+hipError_t ensure_slot(ftl_server* s, Slot& sl, size_t nsteps, size_t nspans, size_t nruns) {
   hipError_t e;
   if (nsteps > sl.steps_cap) {
     const size_t cap = std::max(nsteps, sl.steps_cap * 2);
@@ -221,17 +221,17 @@ hipError_t ensure_slot(ftl_server* s, Slot& sl, size_t nsteps, size_t ntiles, si
       return hipErrorOutOfMemory;
     sl.steps_cap = cap;
   }
-  if (ntiles > sl.tiles_cap) {
-    const size_t cap = std::max(ntiles, sl.tiles_cap * 3 / 2);
-    if (sl.h_tiles) (void)hipHostFree(sl.h_tiles);
-    if (sl.d_tiles) ftl_dev_free(s->ctx, sl.d_tiles, sl.tiles_cap * sizeof(FtlTile));
-    sl.h_tiles = nullptr;
-    sl.d_tiles = nullptr;
-    sl.tiles_cap = 0;
-    if ((e = hipHostMalloc((void**)&sl.h_tiles, cap * sizeof(FtlTile), kHostUpload)) != hipSuccess) return e;
-    if (ftl_dev_alloc(s->ctx, (void**)&sl.d_tiles, cap * sizeof(FtlTile), "server tiles") != FTL_OK)
+  if (nspans > sl.spans_cap) {
+    const size_t cap = std::max(nspans, sl.spans_cap * 2);
+    if (sl.h_spans) (void)hipHostFree(sl.h_spans);
+    if (sl.d_spans) ftl_dev_free(s->ctx, sl.d_spans, sl.spans_cap * sizeof(FtlSpan));
+    sl.h_spans = nullptr;
+    sl.d_spans = nullptr;
+    sl.spans_cap = 0;
+    if ((e = hipHostMalloc((void**)&sl.h_spans, cap * sizeof(FtlSpan), kHostUpload)) != hipSuccess) return e;
+    if (ftl_dev_alloc(s->ctx, (void**)&sl.d_spans, cap * sizeof(FtlSpan), "server spans") != FTL_OK)
       return hipErrorOutOfMemory;
-    sl.tiles_cap = cap;
+    sl.spans_cap = cap;
   }
   if (nsteps > sl.flags_cap) {
     const size_t cap = std::max(nsteps, sl.flags_cap * 2);
@@ -291,31 +291,31 @@ ftl_status launch_batch(ftl_server* s, Slot& sl) {
     }
     g->time = q.r.time;
   }
-  std::vector<FtlTile> tiles;
+  std::vector<FtlSpan> spans;
   std::vector<ClassRun> runs;
-  ftl_status st = ftl_plan_tiles(s->ctx, grooms.data(), (int)n, tiles, runs);
+  ftl_status st = ftl_plan_tiles(s->ctx, grooms.data(), (int)n, spans, runs);
   if (st != FTL_OK) return st;
-  hipError_t e = ensure_slot(s, sl, n, tiles.size(), runs.size());
+  hipError_t e = ensure_slot(s, sl, n, spans.size(), runs.size());
   if (e != hipSuccess) return ftl_hip_fail(e, "server slot buffers");
   for (size_t i = 0; i < n; ++i) {
     if ((st = ftl_make_step(grooms[i], dts[i], &sl.h_steps[i])) != FTL_OK) return st;
     sl.h_flags[i] = 0u;
     sl.h_steps[i].status = sl.d_flags + i;
   }
-  std::memcpy(sl.h_tiles, tiles.data(), tiles.size() * sizeof(FtlTile));
+  if (!spans.empty()) std::memcpy(sl.h_spans, spans.data(), spans.size() * sizeof(FtlSpan));
   if (FTL_KERNEL_UPLOAD) {
     if ((e = ftl_launch_upload(sl.h_steps, sl.d_steps, n * sizeof(FtlStep), sl.s)) != hipSuccess)
       return ftl_hip_fail(e, "server descriptor upload");
-    if ((e = ftl_launch_upload(sl.h_tiles, sl.d_tiles, tiles.size() * sizeof(FtlTile), sl.s)) != hipSuccess)
-      return ftl_hip_fail(e, "server tile upload");
+    if ((e = ftl_launch_upload(sl.h_spans, sl.d_spans, spans.size() * sizeof(FtlSpan), sl.s)) != hipSuccess)
+      return ftl_hip_fail(e, "server span upload");
   } else {
     if ((e = hipMemcpyAsync(sl.d_steps, sl.h_steps, n * sizeof(FtlStep), hipMemcpyHostToDevice, sl.s)) != hipSuccess)
       return ftl_hip_fail(e, "server descriptor upload");
-    if ((e = hipMemcpyAsync(sl.d_tiles, sl.h_tiles, tiles.size() * sizeof(FtlTile), hipMemcpyHostToDevice, sl.s)) !=
-        hipSuccess)
-      return ftl_hip_fail(e, "server tile upload");
+    if (!spans.empty() && (e = hipMemcpyAsync(sl.d_spans, sl.h_spans, spans.size() * sizeof(FtlSpan),
+                                              hipMemcpyHostToDevice, sl.s)) != hipSuccess)
+      return ftl_hip_fail(e, "server span upload");
   }
-  if ((e = ftl_enqueue_runs(runs, sl.d_steps, sl.d_tiles, sl.s, &sl.side, &sl.side_ev, sl.fork)) != hipSuccess)
+  if ((e = ftl_enqueue_runs(runs, sl.d_steps, sl.d_spans, sl.s, &sl.side, &sl.side_ev, sl.fork)) != hipSuccess)
     return ftl_hip_fail(e, "server batch launch");
   if ((e = hipEventRecord(sl.done, sl.s)) != hipSuccess) return ftl_hip_fail(e, "hipEventRecord");
   return FTL_OK;
@@ -562,10 +562,10 @@ extern "C" void ftl_server_destroy(ftl_server* s) {
     if (sl.fork) (void)hipEventDestroy(sl.fork);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.h_steps) (void)hipHostFree(sl.h_steps);
-    if (sl.h_tiles) (void)hipHostFree(sl.h_tiles);
+    if (sl.h_spans) (void)hipHostFree(sl.h_spans);
     if (sl.h_flags) (void)hipHostFree(sl.h_flags);
     ftl_dev_free(s->ctx, sl.d_steps, sl.steps_cap * sizeof(FtlStep));
-    ftl_dev_free(s->ctx, sl.d_tiles, sl.tiles_cap * sizeof(FtlTile));
+    ftl_dev_free(s->ctx, sl.d_spans, sl.spans_cap * sizeof(FtlSpan));
   }
   if (started) {
     std::lock_guard<std::mutex> lk(s->ctx->mu);

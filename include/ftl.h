@@ -268,20 +268,20 @@ ftl_status ftl_groom_root_frames(ftl_groom* g, float* roots, float* normals);
 /* -- batched steps --------------------------------------------------------- */
 
 /* Steps n grooms of one context as one batch: the grooms are cut into 64-lane
-   tiles, and one tile-list kernel per launch class present steps them (the
-   classes run side by side).  dt: one value per groom.  For repeated frames over the same
+   tiles, and one kernel per launch class present steps the tiles of its
+   grooms (the classes run side by side).  dt: one value per groom.  For repeated frames over the same
    grooms, ftl_batch_create / ftl_batch_launch avoid re-planning. */
 ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int n, const float* dt,
                           void* stream);
 
 /* Device scratch of ftl_batch_step_noalloc: per GPU, held until the process
    ends.  It holds the batch's descriptors (288 B per groom, rounded up to
-   256 B) and its tile list (16 B per 64-lane tile). */
+   256 B) and its span list (16 B per groom): about 13,800 grooms. */
 #define FTL_NOALLOC_SCRATCH_BYTES (4u << 20)
 /* ftl_batch_step without any allocation, for callers that cannot allocate
    (code under HIP graph capture of their own, tight serving loops, harnesses
    that forbid hipMalloc).  No counterpart in the reference.  The descriptors
-   and tiles go to a fixed device array instead of context-owned buffers;
+   and spans go to a fixed device array instead of context-owned buffers;
    batches that do not fit it (FTL_NOALLOC_SCRATCH_BYTES) are refused with
    FTL_ERR_INVALID_ARGUMENT, and a groom whose wind lattice is not on the
    device yet (it is placed by a first ftl_groom_step or ftl_batch_step) is
@@ -293,7 +293,7 @@ ftl_status ftl_batch_step(ftl_ctx* ctx, ftl_groom* const* grooms, int n, const f
 ftl_status ftl_batch_step_noalloc(ftl_ctx* ctx, ftl_groom* const* grooms, int n, const float* dt, void* stream);
 
 /* A fixed set of grooms stepped as one frame, captured once into a HIP graph
-   (parameter upload + one tile-list kernel per launch class) and replayed by
+   (parameter upload + one kernel per launch class) and replayed by
    ftl_batch_launch. */
 ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, int n, ftl_batch** out);
 void ftl_batch_destroy(ftl_batch* b);

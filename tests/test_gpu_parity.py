@@ -407,19 +407,23 @@ def test_collider_reach_bound_edges(ctx, lanes):
 
 
 def test_tuned_lane_tables():
-    """The shipped tuning file loads both lanes tables: plain steps with wind
-    take their own choice (16 points: 2 lanes with wind, 4 without), and an
-    explicit lanes setting or a ragged point count does not depend on them."""
+    """The shipped tuning file loads both lanes tables and both class tables
+    (plain steps with wind take their own), and a ragged point count does not
+    depend on them."""
+    import json
     import os
     path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "barbu_amd",
                         "tuned_gfx950.json")
+    with open(path) as f:
+        table = json.load(f)
     c = ba.Context(0)
     try:
         c.autotune(path)  # matches the device: loads, no timing
-        assert c.lanes_for(16) == 4 and c.lanes_for(16, wind=True) == 2
-        for n in (4, 8, 32, 64):
-            assert c.lanes_for(n) == c.lanes_for(n, wind=True)
-        assert c.lanes_for(32) == 2 and c.lanes_for(64) == 4
+        for n in (4, 8, 16, 32, 64):
+            assert c.lanes_for(n) == table["lanes"][str(n)]
+            assert c.lanes_for(n, wind=True) == table["lanes_wind"][str(n)]
+            assert c.class_for(n) == table["class"][str(n)]
+            assert c.class_for(n, True) == table["class_wind"][str(n)]
         assert c.lanes_for(20) == 2 and c.lanes_for(63) == 4  # pipelined guarded classes
     finally:
         c.close()

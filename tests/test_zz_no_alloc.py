@@ -1,5 +1,5 @@
 """ftl_batch_step_noalloc (include/ftl.h): the batched step whose descriptors
-and tile list live in a fixed device array instead of buffers the library
+and span list live in a fixed device array instead of buffers the library
 allocates.  Its results must equal ftl_batch_step's bit for bit (the bar of
 the batched path, which the other GPU tests tie to the FP32 oracle), on two
 calls in a row, the second with a smaller batch (the array keeps the first
@@ -81,20 +81,37 @@ def test_noalloc_matches_batch_step_two_calls(ctx):
 
 
 def test_noalloc_refuses_a_batch_beyond_its_scratch(ctx):
-    """A groom of 2-point strands runs 64 strands per tile; enough strands for
-    more tiles than the array holds is refused, and nothing is stepped."""
-    tiles_max = (4 << 20) // 16
-    S = (tiles_max + 1) * 64
-    rng = np.random.default_rng(3)
-    roots = rng.normal(size=(S, 3)).astype(np.float32)
-    normals = roots / np.linalg.norm(roots, axis=1, keepdims=True)
-    g = ba.reference_groom_device(ctx, roots, normals, npoints=2, maxlength=0.1, seed=1)
+    """The array holds 288 B of descriptor and a 16 B span per groom (about
+    13,800 grooms): a batch of more grooms is refused before anything is
+    planned or stepped."""
+    pos, vel, _ = ba.sphere_groom(64, 8, radius=1.0, length=0.2, seed=4)
+    g = ba.Groom(ctx, pos, vel)
     before = g.download()
-    st = step_noalloc(ctx, [g], np.array([DT], np.float32))
+    n = (4 << 20) // (288 + 16) + 1
+    st = step_noalloc(ctx, [g] * n, np.full(n, DT, np.float32))
     assert st == _ffi.FTL_ERR_INVALID_ARGUMENT
     assert b"no-allocation scratch" in ba.lib().ftl_last_error()
     after = g.download()
     assert np.array_equal(before[0].view(np.uint32), after[0].view(np.uint32))
-    # the same groom steps fine through the allocating entry point
-    ba.batch_step(ctx, [g], np.array([DT], np.float32))
     g.close()
+
+
+def test_noalloc_groom_of_many_tiles(ctx):
+    """The scratch holds one record per groom, whatever its tile count: a
+    groom of 16.8M two-point strands (262,145 tiles of 64 strands) steps
+    bit-identically to ftl_batch_step."""
+    S = (262144 + 1) * 64
+    rng = np.random.default_rng(3)
+    roots = rng.normal(size=(S, 3)).astype(np.float32)
+    normals = roots / np.linalg.norm(roots, axis=1, keepdims=True)
+    a = ba.reference_groom_device(ctx, roots, normals, npoints=2, maxlength=0.1, seed=1)
+    b = ba.reference_groom_device(ctx, roots, normals, npoints=2, maxlength=0.1, seed=1)
+    dt = np.array([DT], np.float32)
+    ba.batch_step(ctx, [a], dt)
+    assert step_noalloc(ctx, [b], dt) == 0
+    pa, va = a.download()
+    pb, vb = b.download()
+    assert_bitexact(pb, pa, "positions")
+    assert_bitexact(vb, va, "velocities")
+    a.close()
+    b.close()
