@@ -313,6 +313,51 @@ def test_alone_equals_batched(ctx):
         assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
 
 
+def test_spans_many_grooms_one_class(ctx):
+    """40 grooms of one launch class with 1 to 40 tiles each (some with a
+    partial last tile), in shuffled order: each workgroup of the class kernel
+    finds its groom by a binary search over the class's spans.  Every groom
+    equals its own plain step bit for bit, through ftl_batch_step and through
+    a captured batch."""
+    rng = np.random.default_rng(21)
+    spt = 64 // ctx.lanes_for(16)
+    shapes = []
+    for t in rng.permutation(np.arange(1, 41)):
+        S = int(spt * t - (int(rng.integers(1, spt)) if t % 3 == 0 else 0))
+        shapes.append(S)
+    states = [ba.sphere_groom(S, 16, radius=1.0, length=0.25, seed=300 + i) for i, S in enumerate(shapes)]
+    params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0)
+    sph = np.array([[0, 0, 0, 1.0], [0.2, 1.1, 0.0, 0.3]], np.float32)
+
+    def make():
+        out = []
+        for pos, vel, _ in states:
+            g = ba.Groom(ctx, pos, vel)
+            g.set_params(params)
+            g.set_colliders(sph)
+            out.append(g)
+        return out
+
+    alone, eager, graph = make(), make(), make()
+    b = ba.Batch(ctx, graph)
+    ntiles, _ = b.info()
+    assert ntiles == sum(-(-S // spt) for S in shapes)
+    for k in range(2):
+        for g in alone:
+            g.step(DT)
+        ba.batch_step(ctx, eager, [DT] * len(eager))
+        b.launch([DT] * len(graph))
+    for i, (ga, ge, gg) in enumerate(zip(alone, eager, graph)):
+        pa, va = ga.download()
+        for gx, label in ((ge, "batch_step"), (gg, "graph")):
+            px, vx = gx.download()
+            assert_bitexact(px, pa, "%s groom %d (%d strands) positions" % (label, i, shapes[i]))
+            assert_bitexact(vx, va, "%s groom %d (%d strands) velocities" % (label, i, shapes[i]))
+    b.close()
+    for g in alone + eager + graph:
+        g.close()
+
+
 def test_skinned_roots(ctx):
     sc = ba.skinned_scene(3000, 16, ntris=2000, nbones=16, seed=2)
     skin = ba.Skin(ctx, sc["rest"], sc["joints"], sc["weights"], sc["tris"], sc["nbones"])
