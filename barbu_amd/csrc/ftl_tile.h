@@ -160,20 +160,26 @@ template <int M, int G, bool GUARD>
 __device__ __forceinline__ void flush_chunk(float4* __restrict__ dst, int strand0, int nstr, int N, int c,
                                             const float4* __restrict__ stage, int lane, int gsel = -1) {
   constexpr int C = Tile<M, G, GUARD>::C;
+  // Row k holds, in lane ln, point ln % C of the chunk of owner lane
+  // r = k * (64 / C) + ln / C.  64 / C is a multiple of G and of C, so from row
+  // to row only the strand moves, by RS strands, and the LDS slot by 64: the
+  // lane's address and slot are formed once and stepped.
+  static_assert((FTL_WAVE / C) % G == 0 && (FTL_WAVE / C) % C == 0, "rows of a chunk hold whole strands");
+  constexpr int RS = FTL_WAVE / C / G;
   // Recomputed here (opaque lane): keeping the load phase's addresses alive
   // until the stores would cost registers for the whole step.
   int ln = lane;
   asm volatile("" : "+v"(ln));
+  const int r0 = ln / C, j = ln % C;
+  const int s0 = strand0 + r0 / G;
+  const int cp = (r0 % G) * M + c * C + j;
+  const bool mine = (!GUARD || cp < N) && (gsel < 0 || r0 % G == gsel);
+  float4* __restrict__ d0 = dst + ((size_t)s0 * N + cp);
+  const float4* __restrict__ t0 = stage + slot<C>(r0, j);
   tile_sync();
 #pragma unroll
-  for (int k = 0; k < C; ++k) {
-    const int q = k * FTL_WAVE + ln;
-    const int r = q / C;
-    const int j = q % C;
-    const int s = strand0 + r / G;
-    const int cp = (r % G) * M + c * C + j;
-    if (s < nstr && (!GUARD || cp < N) && (gsel < 0 || r % G == gsel)) dst[(size_t)s * N + cp] = stage[slot<C>(r, j)];
-  }
+  for (int k = 0; k < C; ++k)
+    if (mine && s0 + k * RS < nstr) d0[(size_t)(k * RS) * N] = t0[k * FTL_WAVE];
   tile_sync();
 }
 
