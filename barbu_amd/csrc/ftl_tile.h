@@ -293,6 +293,15 @@ __device__ __forceinline__ void sweep(const FtlStep& A, const VelArgs& V, const 
   if (!FINAL) collide_sweep<M>(S, p, true, false, GUARD ? nl : M, cm);
 }
 
+// The value of the previous lane of the same row of 16 lanes (DPP row_shr:1,
+// one vector instruction; the first lane of a row reads 0).  The lanes of a
+// strand (G = 2 or 4) never straddle a row, so lane g > 0 of a strand reads
+// lane g - 1, and lane 0 of a strand, which never uses the value, reads
+// another strand's lane or 0.
+__device__ __forceinline__ float from_prev_lane(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xf, 0xf, true));
+}
+
 // One time step of the lane pipeline (G > 1): every lane runs the same code;
 // `active` lanes perform iteration t - g, `fin` lanes are in their last
 // iteration and also write their final velocities (lanes finish at different
@@ -585,19 +594,18 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     // No lane is in its last iteration before time step K - 1.
     // GUARD: the last lane of a strand finishes its tip (slot nl - 1) in its
     // last sweep; the other lanes keep the post-drain path below.
+    static_assert(16 % G == 0, "a strand's lanes lie in one row of 16 lanes");
     const int tip = (GUARD && g == G - 1) ? nl - 1 : -1;
     bool tbad = false;
     for (int t = 0; t < K - 1; ++t) {
-      const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
-                           __shfl_up(last_pre.z, 1, G));
+      const float3 li = f3(from_prev_lane(last_pre.x), from_prev_lane(last_pre.y), from_prev_lane(last_pre.z));
       // (lane 0 of a strand never reads its lead: its slot 0 is the root)
       const float3 lead = li;
       sweep_pipe<M, G, false, GUARD>(A, V, S, p, l, lead, g, t >= g, false, -1, pp, stage, strand0, lane, u0,
                                      last_pre, cfirst, clast, cm, N, tip, &tbad);
     }
     for (int t = K - 1; t < K + G - 1; ++t) {
-      const float3 li = f3(__shfl_up(last_pre.x, 1, G), __shfl_up(last_pre.y, 1, G),
-                           __shfl_up(last_pre.z, 1, G));
+      const float3 li = f3(from_prev_lane(last_pre.x), from_prev_lane(last_pre.y), from_prev_lane(last_pre.z));
       const float3 lead = li;
       const int it = t - g;
       sweep_pipe<M, G, true, GUARD>(A, V, S, p, l, lead, g, it >= 0 && it < K, it == K - 1, t - (K - 1), pp,
