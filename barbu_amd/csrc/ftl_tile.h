@@ -322,8 +322,6 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, c
                                            int lane, float3 u0, float3& last_pre, float3& cfirst, float3& clast,
                                            unsigned cm, int NS = M * G, int tip = -1, bool* tbad = nullptr) {
   constexpr int C = Tile<M, G, GUARD>::C;
-  float sg = A.sigma;
-  asm volatile("" : "+v"(sg));
   // The final steps' velocity constants in vector registers (vector-only
   // operands issue at twice the rate of a scalar operand).
   float dl = V.delta;
@@ -337,7 +335,7 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, c
     // The point is updated in its own registers under the lane mask (a select
     // would be three vector instructions with a scalar mask operand); its
     // leader is the previous point before that point's collision.
-    if (upd) ftl_point(p[j], j == 0 ? lead : p[j - 1], sg * l.get(j));
+    if (upd) ftl_point(p[j], j == 0 ? lead : p[j - 1], l.get(j));  // l holds sigma * rest length
     const float3 pn = p[j];
     if (FINS) {
       const float3 c = sub3(pn, q);
@@ -539,6 +537,19 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     }
   }
 
+  // ---- rest lengths scaled once per tile (pipelined forms) ---------------
+  // Every sweep uses sigma * l[j]: formed here once, the same product each
+  // iteration formed before.  With sigma == 1 it is l[j] itself; otherwise
+  // the position store re-reads the rest lengths from the step's input.  (The
+  // one-lane forms keep their lengths in LDS and multiply per point: the
+  // store's re-read costs them registers they do not have.)
+  const bool unit_sigma = G == 1 || A.sigma == 1.0f;
+  if (!unit_sigma) {
+    const float sg = A.sigma;
+#pragma unroll
+    for (int j = 0; j < M; ++j) l.set(j, sg * l.get(j));
+  }
+
   // ---- colliders the wave can reach this step ----------------------------
   unsigned cm;
   {
@@ -547,7 +558,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     for (int j = 0; j < M; ++j)
       // |sigma * l|: a negative length scale or rest length still places a
       // point that far from its leader.
-      if (j >= (G == 1 ? 1 : 0) && (!GUARD || j < nl)) L += fabsf(A.sigma * l.get(j));
+      if (j >= (G == 1 ? 1 : 0) && (!GUARD || j < nl)) L += fabsf(G == 1 ? A.sigma * l.get(j) : l.get(j));
     float3 root = p[0];
     if (G > 1) {
       // strand total and root, from the lanes of the strand
@@ -639,10 +650,20 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
 #pragma unroll
   for (int c = 0; c < M / C; ++c) {
     if (GUARD && G == 1 && c * C >= nl) continue;
+    float w[C];
+    if (unit_sigma) {
+#pragma unroll
+      for (int j = 0; j < C; ++j) w[j] = l.get(c * C + j);
+    } else {
+      // the unscaled rest lengths, from the input (positions are written
+      // below); a guarded lane's slots past the strand read its last point
+#pragma unroll
+      for (int j = 0; j < C; ++j) w[j] = pp[GUARD ? min(c * C + j, nl - 1) : c * C + j].w;
+    }
 #pragma unroll
     for (int j = 0; j < C; ++j) {
       const float3 q = p[c * C + j];
-      stage[slot<C>(lane, j)] = make_float4(q.x, q.y, q.z, l.get(c * C + j));
+      stage[slot<C>(lane, j)] = make_float4(q.x, q.y, q.z, w[j]);
     }
     flush_chunk<M, G, GUARD>(A.pos, strand0, nstr, N, c, stage, lane);
   }
