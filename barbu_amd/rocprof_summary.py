@@ -134,10 +134,14 @@ def traffic_ids(fetch_db, write_db, copy_bytes, kernels):
     """HBM bytes per dispatch per (kernel, id, workgroups) for the named
     kernels, scaled by the copy kernel's reading; and their sum (one dispatch
     of each group: e.g. one batched frame)."""
-    f = pmc_by_shape(fetch_db)
-    w = pmc_by_shape(write_db)
-    cf = [v[0]["FETCH_SIZE"] for k, v in f.items() if k[0] == COPY_KERNEL][0] * 1024.0
-    cw = [v[0]["WRITE_SIZE"] for k, v in w.items() if k[0] == COPY_KERNEL][0] * 1024.0
+    # The two runs are separate processes, whose code object ids may differ:
+    # a launch is matched across them by kernel, workgroup count and VGPRs.
+    def by_launch(res):
+        return {(k[0], k[2], v[3]): (k[1], v[0]) for k, v in res.items()}
+    f = by_launch(pmc_by_shape(fetch_db))
+    w = by_launch(pmc_by_shape(write_db))
+    cf = [v[1]["FETCH_SIZE"] for k, v in f.items() if k[0] == COPY_KERNEL][0] * 1024.0
+    cw = [v[1]["WRITE_SIZE"] for k, v in w.items() if k[0] == COPY_KERNEL][0] * 1024.0
     kf, kw = copy_bytes / cf, copy_bytes / cw
     out = ["copy kernel: FETCH_SIZE %.0f KiB, WRITE_SIZE %.0f KiB for %d B read + %d B written -> scale "
            "fetch x%.4f, write x%.4f" % (cf / 1024, cw / 1024, copy_bytes, copy_bytes, kf, kw)]
@@ -145,12 +149,12 @@ def traffic_ids(fetch_db, write_db, copy_bytes, kernels):
     for k in sorted(set(f) & set(w)):
         if k[0] not in kernels:
             continue
-        rb = f[k][0]["FETCH_SIZE"] * 1024.0 * kf
-        wb = w[k][0]["WRITE_SIZE"] * 1024.0 * kw
+        rb = f[k][1]["FETCH_SIZE"] * 1024.0 * kf
+        wb = w[k][1]["WRITE_SIZE"] * 1024.0 * kw
         tr += rb
         tw += wb
         out.append("%-26s id %3d wgs %7d  read %.4g B  write %.4g B  total %.4g B per dispatch" %
-                   (k[0], k[1], k[2], rb, wb, rb + wb))
+                   (k[0], f[k][0], k[1], rb, wb, rb + wb))
     out.append("sum (one dispatch of each): read %.4g B  write %.4g B  total %.4g B" % (tr, tw, tr + tw))
     return "\n".join(out), (tr, tw)
 
