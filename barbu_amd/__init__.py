@@ -13,7 +13,7 @@ from ._ffi import (FTL_ROOT_MODEL, FTL_ROOT_REST, FTL_ROOT_SKINNED, FTL_SKIN_DQ,
                    FTL_VELOCITY_DFTL,
                    FTL_VELOCITY_SHIPPED, FTL_MAX_COLLIDERS, FTL_MAX_POINTS, FTL_WIND_GRID,
                    FTL_OK, FTL_ERR_INVALID_ARGUMENT, FTL_ERR_DEVICE, FTL_ERR_OUT_OF_MEMORY, FTL_ERR_BUDGET,
-                   FTL_ERR_NOT_INITIALIZED, FTL_ERR_NONFINITE, FTL_ERR_UNSUPPORTED, FTL_ERR_BUSY,
+                   FTL_ERR_NOT_INITIALIZED, FTL_ERR_NONFINITE, FTL_ERR_UNSUPPORTED, FTL_ERR_BUSY, FTL_ERR_CANCELLED,
                    FtlError, Params, check, lib)
 
 __all__ = ["Context", "Groom", "Skin", "Batch", "Server", "make_requests", "set_request", "Params", "reference_params", "FtlError",

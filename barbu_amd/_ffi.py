@@ -14,6 +14,7 @@ FTL_ERR_NOT_INITIALIZED = 5
 FTL_ERR_NONFINITE = 6
 FTL_ERR_UNSUPPORTED = 7
 FTL_ERR_BUSY = 8
+FTL_ERR_CANCELLED = 9
 
 FTL_VELOCITY_SHIPPED = 0
 FTL_VELOCITY_DFTL = 1

@@ -109,7 +109,8 @@ struct ftl_server {
   std::list<Req> pending;
   std::unordered_set<uint64_t> open;
   // tickets that completed without a step (FTL_ERR_NONFINITE: quarantined
-  // creature), until ftl_server_wait reports them
+  // creature; FTL_ERR_CANCELLED: removed or not admitted), until
+  // ftl_server_wait reports them
   std::unordered_map<uint64_t, ftl_status> failed;
   uint64_t next_ticket = 1;
   uint64_t work_gen = 0;  // bumped whenever something may have become launchable
@@ -156,16 +157,14 @@ ftl_status make_resident(ftl_server* s, Creature& c) {
   return FTL_OK;
 }
 
-// Removes the pending requests of creature id; their tickets complete with
-// status `why` (FTL_OK: cancelled by a removal) (lock held).
-void drop_requests(ftl_server* s, int id, ftl_status why = FTL_OK) {
+// Removes the pending requests of creature id; their tickets complete
+// unstepped, ftl_server_wait reporting `why` (lock held).
+void drop_requests(ftl_server* s, int id, ftl_status why = FTL_ERR_CANCELLED) {
   for (auto it = s->pending.begin(); it != s->pending.end();) {
     if (it->id == id) {
       s->open.erase(it->ticket);
-      if (why != FTL_OK) {
-        s->failed[it->ticket] = why;
-        s->dropped++;
-      }
+      s->failed[it->ticket] = why;
+      if (it->t0 >= s->stats_since) s->dropped++;
       it = s->pending.erase(it);
     } else {
       ++it;
@@ -188,7 +187,8 @@ void admit_waiting(ftl_server* s) {
     s->waiting.pop_front();
     if (st != FTL_OK) {
       // Any other failure is the creature's own: it is dropped (its queued
-      // requests complete as cancelled) and the creatures behind it go on.
+      // requests complete as FTL_ERR_CANCELLED) and the creatures behind it
+      // go on.
       const int id = it->first;
       drop_requests(s, id);
       s->creatures.erase(it);
@@ -339,7 +339,7 @@ void select_batch(ftl_server* s, Slot& sl, Clock::time_point now, Clock::time_po
     if (c.quarantined && !c.inflight && !blocked.count(id)) {
       s->open.erase(it->ticket);
       s->failed[it->ticket] = FTL_ERR_NONFINITE;
-      s->dropped++;
+      if (it->t0 >= s->stats_since) s->dropped++;
       it = s->pending.erase(it);
       s->cv_done.notify_all();
       continue;
@@ -378,13 +378,15 @@ void select_batch(ftl_server* s, Slot& sl, Clock::time_point now, Clock::time_po
 }
 
 // Hands a finished batch's requests back.  Statistics count the requests
-// submitted since the last reset.  A request whose step flagged non-finite
+// submitted since the last reset, and only batches that ran (ok: a batch
+// whose launch or execution failed stops the server, and its requests count
+// as neither stepped nor dropped).  A request whose step flagged non-finite
 // values quarantines its creature.
 void complete_slot(ftl_server* s, Slot& sl, Clock::time_point now, bool ok = true) {
   for (size_t i = 0; i < sl.reqs.size(); ++i) {
     const Req& q = sl.reqs[i];
     const bool bad = ok && sl.h_flags && __atomic_load_n(&sl.h_flags[i], __ATOMIC_ACQUIRE) != 0u;
-    if (q.t0 >= s->stats_since) {
+    if (ok && q.t0 >= s->stats_since) {
       s->lat_us.push_back((float)us_between(q.t0, now));
       s->completed++;
       s->cps += q.cps;
@@ -496,12 +498,12 @@ extern "C" ftl_status ftl_server_create(ftl_ctx* ctx, const ftl_server_config* c
   if (cfg) c = *cfg;
   if (c.max_inflight < 0 || c.max_inflight > kMaxSlots)
     return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "max_inflight must be in [0, %d] (got %d)", kMaxSlots, c.max_inflight);
+  if (c.coalesce_us < 0 || c.coalesce_us > 1000000)
+    return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "coalesce_us must be in [0, 1000000] (got %d)", c.coalesce_us);
   hipError_t e;
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
   ftl_server* s = new ftl_server();
   s->ctx = ctx;
-  if (c.coalesce_us < 0 || c.coalesce_us > 1000000)
-    return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "coalesce_us must be in [0, 1000000] (got %d)", c.coalesce_us);
   s->nslots = c.max_inflight ? c.max_inflight : 2;
   s->batch_cap = c.batch_cap_bytes ? c.batch_cap_bytes : kDefaultBatchCap;
   s->coalesce_us = c.coalesce_us;
@@ -709,8 +711,9 @@ extern "C" ftl_status ftl_server_wait(ftl_server* s, uint64_t ticket, int timeou
   if (f != s->failed.end()) {
     const ftl_status st = f->second;
     s->failed.erase(f);
-    return ftl_fail(st, "request %llu was not stepped: its creature is quarantined (non-finite state)",
-                    (unsigned long long)ticket);
+    return ftl_fail(st, "request %llu was not stepped: %s", (unsigned long long)ticket,
+                    st == FTL_ERR_NONFINITE ? "its creature is quarantined (non-finite state)"
+                                            : "its creature was removed or could not be made resident");
   }
   return FTL_OK;
 }

@@ -49,7 +49,9 @@ enum {
   FTL_ERR_NOT_INITIALIZED = 5, /* handle has no state (Hair::initialized() == false) */
   FTL_ERR_NONFINITE = 6,       /* a step produced NaN/Inf; see ftl_groom_status */
   FTL_ERR_UNSUPPORTED = 7,
-  FTL_ERR_BUSY = 8             /* server: the creature already has a step in flight */
+  FTL_ERR_BUSY = 8,            /* server: the creature already has a step in flight */
+  FTL_ERR_CANCELLED = 9        /* server: the request was dropped unstepped (its creature was removed, or
+                                  could not be made resident) */
 };
 
 /* Velocity stored after a step. */

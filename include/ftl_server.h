@@ -74,7 +74,7 @@ typedef struct ftl_server_stats {
   int resident;           /* creatures on the device */
   int waiting;            /* creatures waiting for memory */
   uint64_t cps_finite;    /* control-point updates of completed steps that stayed finite */
-  uint64_t dropped;       /* requests completed without a step (quarantined creature) */
+  uint64_t dropped;       /* requests completed without a step (quarantined, removed or unadmitted creature) */
   uint64_t failed_admissions; /* waiting creatures dropped because they could not be made resident */
   int quarantined;        /* creatures quarantined now */
 } ftl_server_stats;
@@ -91,8 +91,10 @@ void ftl_server_destroy(ftl_server* s);
 ftl_status ftl_server_add(ftl_server* s, const ftl_groom_desc* desc, const ftl_params* params, int* id,
                           int* resident);
 /* Removes a creature: waits for its step in flight, drops its pending
-   requests (they complete as cancelled), frees its memory and admits waiting
-   creatures that now fit. */
+   requests (they complete unstepped: ftl_server_wait reports them with
+   FTL_ERR_CANCELLED), frees its memory and admits waiting creatures that now
+   fit.  A waiting creature that cannot be made resident for another reason
+   than memory is dropped the same way when its turn comes. */
 ftl_status ftl_server_remove(ftl_server* s, int id);
 /* 1 when the creature is resident, 0 when it waits for memory. */
 ftl_status ftl_server_resident(ftl_server* s, int id, int* resident);
@@ -103,7 +105,8 @@ ftl_status ftl_server_resident(ftl_server* s, int id, int* resident);
 ftl_status ftl_server_submit(ftl_server* s, int n, const int* ids, const ftl_step_request* reqs,
                              uint64_t* tickets);
 /* Waits until the request with this ticket has completed (timeout_ms < 0:
-   no limit).  FTL_ERR_BUSY on timeout. */
+   no limit).  FTL_ERR_BUSY on timeout; FTL_ERR_NONFINITE or
+   FTL_ERR_CANCELLED for a request completed without a step. */
 ftl_status ftl_server_wait(ftl_server* s, uint64_t ticket, int timeout_ms);
 /* Waits until every submitted request of a resident creature has completed
    (requests of creatures waiting for memory stay queued). */

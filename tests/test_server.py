@@ -202,6 +202,37 @@ def test_server_request_errors(ctx):
     srv.close()
 
 
+def test_server_removal_cancels_pending_requests(ctx):
+    """Requests still queued when their creature is removed complete without a
+    step, and ftl_server_wait says so (FTL_ERR_CANCELLED); a creature waiting
+    for memory keeps its requests queued until it is removed the same way.  A
+    coalescing window of a second holds the requests of the resident creature
+    in the queue."""
+    with pytest.raises(ba.FtlError) as e:
+        ba.Server(ctx, coalesce_us=2_000_000)
+    assert e.value.status == ba.FTL_ERR_INVALID_ARGUMENT
+    per = 500 * 16 * 32 + 256
+    srv = ba.Server(ctx, hbm_budget_bytes=per + 1000, coalesce_us=1_000_000, batch_target_bytes=1 << 40)
+    a, res_a = srv.add(*creature(500, 16, 51)[:2], params=params())
+    b, res_b = srv.add(*creature(500, 16, 52)[:2], params=params())
+    assert res_a and not res_b
+    ta = srv.submit([a, a], requests(51, 2))
+    tb = srv.submit([b], requests(52, 1))
+    srv.remove(a)  # admits b; b's request stays queued in the window
+    for t in ta:
+        with pytest.raises(ba.FtlError) as e:
+            srv.wait(t, 60000)
+        assert e.value.status == ba.FTL_ERR_CANCELLED
+    assert srv.resident(b)
+    srv.remove(b)
+    with pytest.raises(ba.FtlError) as e:
+        srv.wait(tb[0], 60000)
+    assert e.value.status == ba.FTL_ERR_CANCELLED
+    st = srv.stats()
+    assert st["dropped"] == 3 and st["completed"] == 0 and st["cps_stepped"] == 0
+    srv.close()
+
+
 def test_server_rejects_invalid_params_before_queueing(ctx):
     """A creature with invalid parameters is refused at ftl_server_add even
     when it would wait for memory, so it cannot block the creatures queued
