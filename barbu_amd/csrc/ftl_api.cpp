@@ -369,11 +369,23 @@ ftl_status ftl_groom_create_uninitialised(ftl_ctx* ctx, const ftl_groom_desc* d,
   return FTL_OK;
 }
 
+// Takes g off its skin's list of bound grooms (ctx->mu held).
+static void unbind_skin_locked(ftl_groom* g) {
+  if (!g->skin) return;
+  std::vector<ftl_groom*>& b = g->skin->bound;
+  b.erase(std::remove(b.begin(), b.end(), g), b.end());
+  g->skin = nullptr;
+}
+
 extern "C" void ftl_groom_destroy(ftl_groom* g) {
   if (!g) return;
   ftl_ctx* ctx = g->ctx;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    unbind_skin_locked(g);
+  }
   const size_t bytes = (size_t)g->nstrands * g->npoints * sizeof(float4);
   ftl_dev_free(ctx, g->d_pos, bytes);
   ftl_dev_free(ctx, g->d_vel, bytes);
@@ -716,6 +728,13 @@ extern "C" void ftl_skin_destroy(ftl_skin* s) {
   ftl_ctx* ctx = s->ctx;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
+  {
+    // Its grooms lose their binding: a step with skinned roots then fails
+    // (no bound skin) instead of reading freed memory.
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (ftl_groom* g : s->bound) g->skin = nullptr;
+    s->bound.clear();
+  }
   const size_t vb = (size_t)s->nverts * 16, tb = (size_t)s->ntris * 16, bb = (size_t)s->nbones * 32,
                mb = (size_t)s->nbones * 48;
   ftl_dev_free(ctx, s->d_rest, vb);
@@ -958,7 +977,14 @@ extern "C" ftl_status ftl_groom_bind_skin(ftl_groom* g, ftl_skin* s, const int* 
   if ((e = hipDeviceSynchronize()) != hipSuccess) return ftl_hip_fail(e, "hipDeviceSynchronize");
   if ((e = hipMemcpy(g->d_bind, b.data(), bytes, hipMemcpyHostToDevice)) != hipSuccess)
     return ftl_hip_fail(e, "hipMemcpy binding");
-  g->skin = s;
+  {
+    std::lock_guard<std::mutex> lk(g->ctx->mu);
+    if (g->skin != s) {
+      unbind_skin_locked(g);
+      g->skin = s;
+      s->bound.push_back(g);
+    }
+  }
   g->root_mode = FTL_ROOT_SKINNED;
   return FTL_OK;
 }

@@ -35,6 +35,8 @@ struct ftl_ctx {
   size_t scratch_spans_bytes = 0;
 };
 
+struct ftl_groom;
+
 struct ftl_skin {
   ftl_ctx* ctx = nullptr;
   int nverts = 0, ntris = 0, nbones = 0;
@@ -51,6 +53,8 @@ struct ftl_skin {
   void* d_nout = nullptr;
   std::vector<float> h_dq, h_mats;
   bool dirty = false, mats_dirty = false;
+  // grooms bound to this skin (ctx->mu): destroying the skin unbinds them
+  std::vector<ftl_groom*> bound;
 };
 
 struct ftl_groom {

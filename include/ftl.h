@@ -256,7 +256,9 @@ ftl_status ftl_skin_download_normals(ftl_skin* s, float* normals);
 ftl_status ftl_dualquat_from_matrix(const float* m3x4, float* dq);
 /* Binds each strand's root to a triangle: tri[s] in [0,ntris), bary[2s..2s+1]
    = (w1, w2) with w0 = 1 - w1 - w2.  Sets the groom's root mode to
-   FTL_ROOT_SKINNED.  The skin must outlive the binding. */
+   FTL_ROOT_SKINNED.  Destroying the skin unbinds its grooms: their next step
+   with skinned roots fails with FTL_ERR_INVALID_ARGUMENT until they are
+   bound again or take another root mode. */
 ftl_status ftl_groom_bind_skin(ftl_groom* g, ftl_skin* s, const int* tri, const float* bary);
 /* The emitter frame of each strand of a skinned groom at the skin's current
    pose: its root on the bound triangle (the root the next step uses) and,

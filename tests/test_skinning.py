@@ -341,3 +341,34 @@ def test_gpu_lbs_skinned_roots_step_and_root_frames(ctx):
     regrown.close()
     g.close()
     skin.close()
+
+
+@pytest.mark.gpu
+def test_gpu_skin_destroyed_before_its_grooms(ctx):
+    """Destroying a skin unbinds the grooms bound to it: a step with skinned
+    roots is then refused (no bound skin) rather than reading the freed mesh,
+    another root mode steps on, and a groom bound to a second skin, or
+    destroyed first, detaches cleanly."""
+    sc = ba.skinned_scene(300, 8, ntris=200, nbones=4, seed=21)
+    skin = ba.Skin(ctx, sc["rest"], sc["joints"], sc["weights"], sc["tris"], sc["nbones"])
+    other = ba.Skin(ctx, sc["rest"], sc["joints"], sc["weights"], sc["tris"], sc["nbones"])
+    g = ba.Groom(ctx, sc["pos"], sc["vel"])
+    h = ba.Groom(ctx, sc["pos"], sc["vel"])
+    g.bind_skin(skin, sc["tri"], sc["bary"])
+    h.bind_skin(skin, sc["tri"], sc["bary"])
+    h.bind_skin(other, sc["tri"], sc["bary"])  # moves h to the second skin
+    skin.update()
+    g.step(DT)
+    skin.close()
+    with pytest.raises(ba.FtlError) as e:
+        g.step(DT)
+    assert e.value.status == ba.FTL_ERR_INVALID_ARGUMENT
+    with pytest.raises(ba.FtlError):
+        g.set_root_mode(ba.FTL_ROOT_SKINNED)
+    g.set_root_mode(ba.FTL_ROOT_MODEL)
+    g.step(DT)
+    other.update()
+    h.step(DT)  # still bound to the skin that lives
+    h.close()  # detaches from `other` before it goes
+    other.close()
+    g.close()
