@@ -385,6 +385,8 @@ extern "C" void ftl_groom_destroy(ftl_groom* g) {
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     unbind_skin_locked(g);
+    for (ftl_batch* b : g->batches) ftl_batch_forget_groom_locked(b, g);
+    g->batches.clear();
   }
   const size_t bytes = (size_t)g->nstrands * g->npoints * sizeof(float4);
   ftl_dev_free(ctx, g->d_pos, bytes);

@@ -360,9 +360,15 @@ extern "C" ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, i
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->live_objects++;
+    for (ftl_groom* g : b->grooms) g->batches.push_back(b);
   }
   *out = b;
   return FTL_OK;
+}
+
+void ftl_batch_forget_groom_locked(ftl_batch* b, ftl_groom* g) {
+  for (ftl_groom*& x : b->grooms)
+    if (x == g) x = nullptr;
 }
 
 extern "C" void ftl_batch_destroy(ftl_batch* b) {
@@ -372,6 +378,8 @@ extern "C" void ftl_batch_destroy(ftl_batch* b) {
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->live_objects--;
+    for (ftl_groom* g : b->grooms)
+      if (g) g->batches.erase(std::remove(g->batches.begin(), g->batches.end(), b), g->batches.end());
   }
   delete b;
 }
@@ -384,6 +392,12 @@ extern "C" ftl_status ftl_batch_launch(ftl_batch* b, const float* dt, void* stre
   // The pinned descriptors of buffer k were last read by the frame before the
   // previous one.
   if ((e = hipEventSynchronize(b->done[k])) != hipSuccess) return ftl_hip_fail(e, "hipEventSynchronize");
+  {
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    for (size_t i = 0; i < b->grooms.size(); ++i)
+      if (!b->grooms[i])
+        return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom %zu of the batch was destroyed (destroy the batch)", i);
+  }
   for (size_t i = 0; i < b->grooms.size(); ++i) {
     ftl_status st = ftl_make_step(b->grooms[i], dt[i], &b->h_steps[k][i]);
     if (st != FTL_OK) return st;

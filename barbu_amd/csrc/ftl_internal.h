@@ -77,7 +77,14 @@ struct ftl_groom {
   double time = 0.0;
   ftl_skin* skin = nullptr;
   int lanes = 0;
+  // captured batches that step this groom (ctx->mu): destroying the groom
+  // takes it out of them
+  std::vector<ftl_batch*> batches;
 };
+
+// Takes g out of batch b's groom list (ctx->mu held): the batch's next
+// launch is refused.
+void ftl_batch_forget_groom_locked(ftl_batch* b, ftl_groom* g);
 
 // Render-side expansion object (ftl_render.hip); its tangents are also
 // written by the device-side initialisation (ftl_init.hip).

@@ -302,7 +302,9 @@ ftl_status ftl_batch_step_noalloc(ftl_ctx* ctx, ftl_groom* const* grooms, int n,
 ftl_status ftl_batch_create(ftl_ctx* ctx, ftl_groom* const* grooms, int n, ftl_batch** out);
 void ftl_batch_destroy(ftl_batch* b);
 /* One frame: per-groom dt (n values).  Colliders, transforms, params and time
-   are read from the grooms at call time.  Asynchronous on `stream`. */
+   are read from the grooms at call time.  Asynchronous on `stream`.  Refused
+   with FTL_ERR_INVALID_ARGUMENT once one of the batch's grooms has been
+   destroyed. */
 ftl_status ftl_batch_launch(ftl_batch* b, const float* dt, void* stream);
 /* Tiles (64-lane work units) in the batch and the total number of
    workgroups its kernels launch. */

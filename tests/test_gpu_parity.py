@@ -295,6 +295,26 @@ def test_graph_replays_match_oracle(ctx):
     b.close()
 
 
+def test_batch_refuses_a_destroyed_groom(ctx):
+    """A captured batch whose groom has been destroyed refuses its next launch
+    (rather than stepping freed memory); the batch and the other grooms are
+    then destroyed cleanly, in either order."""
+    pos, vel, _ = ba.sphere_groom(300, 16, seed=8)
+    grooms = [ba.Groom(ctx, pos, vel) for _ in range(3)]
+    b = ba.Batch(ctx, grooms)
+    b.launch([DT] * 3)
+    grooms[1].close()
+    with pytest.raises(ba.FtlError) as e:
+        b.launch([DT] * 3)
+    assert e.value.status == ba.FTL_ERR_INVALID_ARGUMENT
+    b2 = ba.Batch(ctx, [grooms[0], grooms[2]])
+    b2.launch([DT] * 2)
+    b.close()
+    grooms[0].close()
+    b2.close()
+    grooms[2].close()
+
+
 def test_alone_equals_batched(ctx):
     """Each groom's result is the same stepped alone and in the mix."""
     grooms_a, _ = _mixed_grooms(ctx, seed=9)
