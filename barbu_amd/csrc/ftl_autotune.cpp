@@ -31,20 +31,48 @@ static bool candidate(int cls, int n) {
   return !k.guarded && k.m * k.g == n && k.m <= 32;
 }
 
+// The tables are flat JSON written by save_cache (or any writer that keeps
+// that structure, whatever its spacing).  value_at: the position of the value
+// of member `key` (a quoted name followed by ':') within [from, to), or npos.
+static size_t value_at(const std::string& s, const std::string& key, size_t from, size_t to) {
+  auto blank = [](char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r'; };
+  for (size_t k = s.find(key, from); k != std::string::npos && k < to; k = s.find(key, k + 1)) {
+    size_t i = k + key.size();
+    while (i < s.size() && blank(s[i])) ++i;
+    if (i >= s.size() || s[i] != ':') continue;
+    ++i;
+    while (i < s.size() && blank(s[i])) ++i;
+    if (i < to) return i;
+  }
+  return std::string::npos;
+}
+
+// The object that member `name` holds (the first member of that name whose
+// value is an object: the timing records reuse some names for numbers):
+// [open brace, closing brace), or npos.
+static size_t object_at(const std::string& s, const char* name, size_t* end) {
+  for (size_t v = value_at(s, name, 0, s.size()); v != std::string::npos; v = value_at(s, name, v, s.size())) {
+    if (s[v] != '{') continue;
+    *end = s.find('}', v);
+    return *end == std::string::npos ? std::string::npos : v;
+  }
+  return std::string::npos;
+}
+
 // Reads `"name": { "n": v, ... }` into out[n] for the tuned point counts;
 // valid(n, v) filters the values.  Returns the number read.
 template <typename F>
 static int read_table(const std::string& s, const char* name, int* out, F valid) {
-  const size_t at = s.find(name);
+  size_t end = 0;
+  const size_t at = object_at(s, name, &end);
   if (at == std::string::npos) return 0;
-  const size_t end = s.find('}', at);
   int found = 0;
   char key[32];
   for (int n : kTunedPoints) {
-    std::snprintf(key, sizeof key, "\"%d\": ", n);
-    const size_t k = s.find(key, at);
-    if (k == std::string::npos || k > end) continue;
-    const int v = std::atoi(s.c_str() + k + std::strlen(key));
+    std::snprintf(key, sizeof key, "\"%d\"", n);
+    const size_t k = value_at(s, key, at, end);
+    if (k == std::string::npos) continue;
+    const int v = std::atoi(s.c_str() + k);
     if (valid(n, v)) {
       out[n] = v;
       ++found;
@@ -59,18 +87,20 @@ static bool load_cache(ftl_ctx* ctx, const char* path) {
   std::stringstream ss;
   ss << f.rdbuf();
   const std::string s = ss.str();
-  if (s.find("\"arch\": \"gfx950\"") == std::string::npos) return false;
+  // the tables hold for this architecture and CU count only
+  const size_t arch = value_at(s, "\"arch\"", 0, s.size());
+  if (arch == std::string::npos || s.compare(arch, 8, "\"gfx950\"") != 0) return false;
+  const size_t ncu = value_at(s, "\"ncu\"", 0, s.size());
+  if (ncu == std::string::npos || std::atoi(s.c_str() + ncu) != ctx->ncu) return false;
   char key[32];
-  std::snprintf(key, sizeof key, "\"ncu\": %d,", ctx->ncu);
-  if (s.find(key) == std::string::npos) return false;
-  const size_t ct = s.find("\"class_ns_per_cp\"");
+  size_t ct_end = 0;
+  const size_t ct = object_at(s, "\"class_ns_per_cp\"", &ct_end);
   if (ct != std::string::npos) {
     for (int c = 0; c < FTL_NCLASSES; ++c) {
-      std::snprintf(key, sizeof key, "\"%d\": ", c);
-      const size_t k = s.find(key, ct);
-      const size_t end = s.find('}', ct);
-      if (k == std::string::npos || k > end) continue;
-      const float v = (float)std::atof(s.c_str() + k + std::strlen(key));
+      std::snprintf(key, sizeof key, "\"%d\"", c);
+      const size_t k = value_at(s, key, ct, ct_end);
+      if (k == std::string::npos) continue;
+      const float v = (float)std::atof(s.c_str() + k);
       if (v > 0.0f && v < 1e3f) ctx->cls_ns_cp[c] = v;
     }
   }
@@ -169,12 +199,14 @@ extern "C" ftl_status ftl_autotune(ftl_ctx* ctx, const char* cache_path, int for
         if ((st = ftl_make_step(g, 1.0f / 60.0f, &A)) != FTL_OK) break;
         // Timed on a settled state: the first steps from the uploaded groom
         // (strands still straight) collide far less than later ones.
-        for (int w = 0; w < 24; ++w) ftl_launch_plain(cls, A, nullptr);
+        // (a launch that fails must not look fast: the tuning stops on it)
+        e = hipSuccess;
+        for (int w = 0; w < 24 && e == hipSuccess; ++w) e = ftl_launch_plain(cls, A, nullptr);
         const int reps = 10;
-        (void)hipEventRecord(t0, nullptr);
-        for (int r = 0; r < reps; ++r) ftl_launch_plain(cls, A, nullptr);
-        (void)hipEventRecord(t1, nullptr);
-        if ((e = hipEventSynchronize(t1)) != hipSuccess) {
+        if (e == hipSuccess) e = hipEventRecord(t0, nullptr);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = ftl_launch_plain(cls, A, nullptr);
+        if (e == hipSuccess) e = hipEventRecord(t1, nullptr);
+        if (e != hipSuccess || (e = hipEventSynchronize(t1)) != hipSuccess) {
           st = ftl_hip_fail(e, "autotune timing");
           break;
         }

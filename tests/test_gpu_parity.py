@@ -471,27 +471,31 @@ def test_collider_reach_bound_edges(ctx, lanes):
     run_compare(ctx, pos, vel, params, spheres, 3, lanes=lanes, label="reach G=%d" % lanes)
 
 
-def test_tuned_lane_tables():
+def test_tuned_lane_tables(tmp_path):
     """The shipped tuning file loads both lanes tables and both class tables
-    (plain steps with wind take their own), and a ragged point count does not
-    depend on them."""
+    (plain steps with wind take their own), also rewritten compactly (no
+    blanks, other key order), and a ragged point count does not depend on
+    them."""
     import json
     import os
     path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "barbu_amd",
                         "tuned_gfx950.json")
     with open(path) as f:
         table = json.load(f)
-    c = ba.Context(0)
-    try:
-        c.autotune(path)  # matches the device: loads, no timing
-        for n in (4, 8, 16, 32, 64):
-            assert c.lanes_for(n) == table["lanes"][str(n)]
-            assert c.lanes_for(n, wind=True) == table["lanes_wind"][str(n)]
-            assert c.class_for(n) == table["class"][str(n)]
-            assert c.class_for(n, True) == table["class_wind"][str(n)]
-        assert c.lanes_for(20) == 2 and c.lanes_for(63) == 4  # pipelined guarded classes
-    finally:
-        c.close()
+    compact = tmp_path / "compact.json"
+    compact.write_text(json.dumps(dict(reversed(list(table.items()))), separators=(",", ":")))
+    for p in (path, str(compact)):
+        c = ba.Context(0)
+        try:
+            c.autotune(p)  # matches the device: loads, no timing
+            for n in (4, 8, 16, 32, 64):
+                assert c.lanes_for(n) == table["lanes"][str(n)]
+                assert c.lanes_for(n, wind=True) == table["lanes_wind"][str(n)]
+                assert c.class_for(n) == table["class"][str(n)]
+                assert c.class_for(n, True) == table["class_wind"][str(n)]
+            assert c.lanes_for(20) == 2 and c.lanes_for(63) == 4  # pipelined guarded classes
+        finally:
+            c.close()
 
 
 @pytest.mark.parametrize("npoints,lanes", [(32, 1), (32, 2), (64, 4), (20, 0), (20, 1), (40, 0), (63, 0)])
