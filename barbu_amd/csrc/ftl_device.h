@@ -82,6 +82,33 @@ __device__ __forceinline__ float rsqrt_c(float x) {
   return y;
 }
 
+// sin (cosine = 0) or cos (cosine = 1) of x, from plain IEEE operations so that
+// the result is reproducible bit for bit: Cody-Waite reduction by pi/2 in three
+// parts (k = rint(x * 2/pi), r = x - k pi/2 with one fma per part), then the
+// Taylor polynomials of sin and cos on |r| <= pi/4 (terms to r^9 and r^10,
+// Horner with fma) and the quadrant k + cosine.  Within 1.5 ulp of the exact
+// value for |x| <= 200 (the range the groom generator uses).
+__device__ __forceinline__ float trig_c(float x, int cosine) {
+  const float k = __builtin_rintf(x * 0.636619747f);
+  float r = __builtin_fmaf(-k, 1.57079637f, x);
+  r = __builtin_fmaf(-k, -4.37113883e-08f, r);
+  r = __builtin_fmaf(-k, -1.71512451e-15f, r);
+  const int q = ((int)k + cosine) & 3;
+  const float r2 = r * r;
+  float ps = __builtin_fmaf(r2, 2.75573188e-06f, -1.98412701e-04f);
+  ps = __builtin_fmaf(r2, ps, 8.33333377e-03f);
+  ps = __builtin_fmaf(r2, ps, -1.66666672e-01f);
+  const float s = __builtin_fmaf(r2 * r, ps, r);
+  float pc = __builtin_fmaf(r2, -2.75573200e-07f, 2.48015876e-05f);
+  pc = __builtin_fmaf(r2, pc, -1.38888892e-03f);
+  pc = __builtin_fmaf(r2, pc, 4.16666679e-02f);
+  pc = __builtin_fmaf(r2, pc, -0.5f);
+  const float c = __builtin_fmaf(r2, pc, 1.0f);
+  return q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
+}
+__device__ __forceinline__ float sin_c(float x) { return trig_c(x, 0); }
+__device__ __forceinline__ float cos_c(float x) { return trig_c(x, 1); }
+
 // One FTL point update, in place (DistanceConstraint, cs_simulation.glsl:110-116):
 //   d = p - lead;  p = lead + (sl * inversesqrt(dot(d, d))) * d
 // with the operations of dot3 / rsqrt_c / fma3 above, in the same order.

@@ -129,10 +129,10 @@ __global__ __launch_bounds__(256) void ftl_init_tangents_kernel(TangentArgs A) {
   } else if (il == last) {
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
     if (A.curly) {
-      const float n = 1.25f * simplex2(sinf(3.0f * dj), cosf(5.0f));
-      cx = cosf(n * 4.0f * kPi);
+      const float n = 1.25f * simplex2(ftl_dev::sin_c(3.0f * dj), ftl_dev::cos_c(5.0f));
+      cx = ftl_dev::cos_c(n * 4.0f * kPi);
       cy = -0.71f * n;
-      cz = sinf(n * 2.7f * kPi);
+      cz = ftl_dev::sin_c(n * 2.7f * kPi);
     }
     out = make_float4(0.2f * (-nr[0] + cx), 0.2f * (-nr[1] + cy), 0.2f * (-nr[2] + cz), 0.0f);
   } else {
@@ -141,10 +141,10 @@ __global__ __launch_bounds__(256) void ftl_init_tangents_kernel(TangentArgs A) {
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
     if (A.curly) {
       const float di = 10.0f * (float)(last - il) / (dist_ab - 1.0f);
-      const float n = di * simplex2(sinf(43.0f * dj), cosf(5.0f * di));
-      cx = -5.8f * (10.7f * cosf(n * kPi));
+      const float n = di * simplex2(ftl_dev::sin_c(43.0f * dj), ftl_dev::cos_c(5.0f * di));
+      cx = -5.8f * (10.7f * ftl_dev::cos_c(n * kPi));
       cy = -5.8f * (-2.3f * n);
-      cz = -5.8f * (20.5f * sinf(n * kPi));
+      cz = -5.8f * (20.5f * ftl_dev::sin_c(n * kPi));
     }
     const float s = 0.1f * (float)il * inv_dist * scale_max_length;
     out = make_float4(s * cx, s * cy, s * cz, s * 0.0f);

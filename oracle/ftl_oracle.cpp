@@ -738,6 +738,34 @@ uint64_t init_draw(uint64_t seed, uint64_t j) {
   return z ^ (z >> 31);
 }
 
+// sin and cos of the curly tangents (hair.cc:300-322 calls std::sin/std::cos,
+// whose last bits differ between libraries): one FP32 definition the device
+// shares, from the mathematics only.  Cody and Waite's reduction by pi/2 in
+// three parts (k = rint(x * 2/pi), r = x - k pi/2, one fma per part; the
+// parts are pi/2 rounded to float, then the rounded remainders), the Taylor
+// series of sin (to r^9) and cos (to r^10) on |r| <= pi/4 with coefficients
+// (-1)^n/n! rounded to float, evaluated by Horner with fma, and the quadrant.
+float trig_f32(float x, int cosine) {
+  const float k = std::nearbyint(x * 0.636619747f);
+  float r = std::fma(-k, 1.57079637f, x);
+  r = std::fma(-k, -4.37113883e-08f, r);
+  r = std::fma(-k, -1.71512451e-15f, r);
+  const int q = (static_cast<int>(k) + cosine) & 3;
+  const float r2 = r * r;
+  float ps = std::fma(r2, 2.75573188e-06f, -1.98412701e-04f);
+  ps = std::fma(r2, ps, 8.33333377e-03f);
+  ps = std::fma(r2, ps, -1.66666672e-01f);
+  const float s = std::fma(r2 * r, ps, r);
+  float pc = std::fma(r2, -2.75573200e-07f, 2.48015876e-05f);
+  pc = std::fma(r2, pc, -1.38888892e-03f);
+  pc = std::fma(r2, pc, 4.16666679e-02f);
+  pc = std::fma(r2, pc, -0.5f);
+  const float c = std::fma(r2, pc, 1.0f);
+  return q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
+}
+float trig_f32_sin(float x) { return trig_f32(x, 0); }
+float trig_f32_cos(float x) { return trig_f32(x, 1); }
+
 // glm::simplex(vec2) (glm/gtc/noise.inl; the GLM tree is not part of the
 // reference), from the published algorithm: McEwan, Sheets, Gustavson,
 // Richardson, "Efficient computational noise in GLSL", JGT 17(2), 2012
@@ -803,6 +831,8 @@ void oracle_reference_groom_f32(int nroots, const float* roots, const float* nor
 }
 
 float oracle_simplex2_f32(float x, float y) { return simplex2_f32(x, y); }
+float oracle_sin_f32(float x) { return trig_f32(x, 0); }
+float oracle_cos_f32(float x) { return trig_f32(x, 1); }
 
 // The tangents of hair.cc:290-328 (curly or not), float4 [nstrands*npoints].
 void oracle_reference_tangents_f32(int nstrands, int npoints, const float* normals, float maxlength, int curly,
@@ -816,10 +846,10 @@ void oracle_reference_tangents_f32(int nstrands, int npoints, const float* norma
     const float* n = normals + 3 * j;
     float c[3] = {0.0f, 0.0f, 0.0f};
     if (curly) {
-      const float k = 1.25f * simplex2_f32(std::sin(3.0f * dj), std::cos(5.0f));
-      c[0] = std::cos(k * 4.0f * kPi);
+      const float k = 1.25f * simplex2_f32(trig_f32_sin(3.0f * dj), trig_f32_cos(5.0f));
+      c[0] = trig_f32_cos(k * 4.0f * kPi);
       c[1] = -0.71f * k;
-      c[2] = std::sin(k * 2.7f * kPi);
+      c[2] = trig_f32_sin(k * 2.7f * kPi);
     }
     for (int d = 0; d < 3; ++d) {
       tan[4 * (size_t)A + d] = 0.15f * n[d];
@@ -832,10 +862,10 @@ void oracle_reference_tangents_f32(int nstrands, int npoints, const float* norma
     for (int i = A + 1; i < B; ++i) {
       if (curly) {
         const float di = 10.0f * static_cast<float>(B - i) / (dist - 1.0f);
-        const float k = di * simplex2_f32(std::sin(43.0f * dj), std::cos(5.0f * di));
-        c[0] = -5.8f * (10.7f * std::cos(k * kPi));
+        const float k = di * simplex2_f32(trig_f32_sin(43.0f * dj), trig_f32_cos(5.0f * di));
+        c[0] = -5.8f * (10.7f * trig_f32_cos(k * kPi));
         c[1] = -5.8f * (-2.3f * k);
-        c[2] = -5.8f * (20.5f * std::sin(k * kPi));
+        c[2] = -5.8f * (20.5f * trig_f32_sin(k * kPi));
       }
       const float s = 0.1f * static_cast<float>(i - A) * inv_dist * scaleMaxLength;
       for (int d = 0; d < 3; ++d) tan[4 * (size_t)i + d] = s * c[d];

@@ -71,6 +71,9 @@ def lib():
         L.oracle_reference_groom_f32.argtypes = [C.c_int, vp, vp, C.c_int, C.c_float, C.c_uint64, vp, vp]
         L.oracle_simplex2_f32.restype = C.c_float
         L.oracle_simplex2_f32.argtypes = [C.c_float, C.c_float]
+        for fn in (L.oracle_sin_f32, L.oracle_cos_f32):
+            fn.restype = C.c_float
+            fn.argtypes = [C.c_float]
         L.oracle_reference_tangents_f32.argtypes = [C.c_int, C.c_int, vp, C.c_float, C.c_int, vp]
         L.oracle_marschner_f32.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, vp, vp]
         L.oracle_marschner_f64.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, vp, vp]
@@ -281,6 +284,15 @@ def reference_groom_f32(roots, normals, npoints, maxlength, seed):
 
 def simplex2_f32(x, y):
     return float(lib().oracle_simplex2_f32(f32(x), f32(y)))
+
+
+def sin_f32(x):
+    """The FP32 sine of the groom generator's curly tangents."""
+    return float(lib().oracle_sin_f32(f32(x)))
+
+
+def cos_f32(x):
+    return float(lib().oracle_cos_f32(f32(x)))
 
 
 def reference_tangents_f32(normals, npoints, maxlength, curly=True):

@@ -10,9 +10,9 @@ Pins of the oracle (CPU):
     permutation ring gives (a wrong modulus or skew constant breaks it);
   * tangents: the root gets 0.15 n, the tip 0.2 (-n + c), the inner points
     s_i c_i with s_i linear in i (exact without curl: 0).
-GPU: positions/velocities and uncurled tangents bit for bit; curled tangents
-within the tolerance libm vs device sinf/cosf leave (relative 1e-4 of the
-largest tangent).
+  * the curly tangents' sine and cosine (one FP32 definition, shared with the
+    device): against float64 within 1.5 ulp, quadrant points, symmetries.
+GPU: positions/velocities and tangents, curled or not, bit for bit.
 """
 import numpy as np
 import pytest
@@ -38,6 +38,25 @@ def test_oracle_matches_host_generator():
         # offsets grow linearly by the per-strand factor r_j in [0.9, 1.1]
         seg = p1[:, 1:, 3].astype(np.float64)
         assert (seg > 0).all() and ((seg / (L / N) > 0.9 - 1e-6) & (seg / (L / N) < 1.1 + 1e-6)).all()
+
+
+def test_trig_against_float64():
+    """The FP32 sine and cosine of the curly tangents: within 1.5 ulp of the
+    float64 values over the generator's range (ulps of max(|f|, 1e-3)), exact
+    at the quadrant points, odd/even, and pi/2-shifted into each other (a
+    wrong reduction constant, quadrant or coefficient sign fails these)."""
+    xs = np.concatenate([np.linspace(-200, 200, 40001), np.random.default_rng(3).uniform(-60, 60, 20000),
+                         np.arange(-64, 65) * (np.pi / 2)]).astype(np.float32)
+    for f, ref in ((orc.sin_f32, np.sin), (orc.cos_f32, np.cos)):
+        got = np.array([f(x) for x in xs], np.float64)
+        want = ref(xs.astype(np.float64))
+        ulp = np.exp2(np.floor(np.log2(np.maximum(np.abs(want), 1e-3))) - 23)
+        assert (np.abs(got - want) / ulp).max() <= 1.5
+    assert orc.sin_f32(0.0) == 0.0 and orc.cos_f32(0.0) == 1.0
+    assert orc.sin_f32(np.float32(np.pi / 2)) == 1.0 and orc.cos_f32(np.float32(np.pi)) == -1.0
+    for x in np.random.default_rng(4).uniform(-50, 50, 200).astype(np.float32):
+        assert orc.sin_f32(-x) == -orc.sin_f32(x) and orc.cos_f32(-x) == orc.cos_f32(x)
+        assert abs(orc.sin_f32(x) - orc.cos_f32(np.float32(x - np.float32(np.pi / 2)))) < 1e-6
 
 
 def test_simplex_properties():
@@ -112,6 +131,5 @@ def test_gpu_tangents_match_oracle(ctx):
         e.reference_tangents(normals, 0.5, curly=True)
         got = e.tangents()
         want = orc.reference_tangents_f32(normals, N, 0.5, True)
-        tol = 1e-4 * max(1e-6, float(np.abs(want).max()))
-        assert np.abs(got - want).max() <= tol, np.abs(got - want).max()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), np.abs(got - want).max()
         e.close()
