@@ -109,6 +109,91 @@ __device__ __forceinline__ float trig_c(float x, int cosine) {
 __device__ __forceinline__ float sin_c(float x) { return trig_c(x, 0); }
 __device__ __forceinline__ float cos_c(float x) { return trig_c(x, 1); }
 
+// asin, acos, exp, log and pow from plain IEEE operations (reproducible bit for
+// bit; within 2 ulp, pow 3 ulp, of the exact values on the Marschner tables'
+// ranges):
+// * asin on [0, 1/2]: a + a^3 P(a^2), P the Taylor series of asin to a^23
+//   (coefficients (2n)! / (4^n (n!)^2 (2n+1)) rounded to float, Horner with
+//   fma); above 1/2, asin a = pi/2 - 2 asin(sqrt((1 - a)/2)); acos from the same
+//   core (pi/2 - asin, 2 asin(...), pi - 2 asin(...)), pi and pi/2 in two parts;
+// * exp: Cody-Waite by ln 2 in two parts, the Taylor series of e^r to r^8,
+//   then the scaling by 2^k (ldexp, one rounding);
+// * log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with
+//   s = (m - 1)/(m + 1) (series to s^11), plus e ln 2 in two parts;
+// * pow(x, y) = exp(y log x) for x >= 0, pow(0, y) = 0.
+__device__ __forceinline__ float asin_core_c(float a) {
+  const float z = a * a;
+  float p = __builtin_fmaf(z, 0.00731252600f, 0.00839033537f);
+  p = __builtin_fmaf(z, p, 0.00976160914f);
+  p = __builtin_fmaf(z, p, 0.0115518011f);
+  p = __builtin_fmaf(z, p, 0.0139648439f);
+  p = __builtin_fmaf(z, p, 0.0173527636f);
+  p = __builtin_fmaf(z, p, 0.0223721582f);
+  p = __builtin_fmaf(z, p, 0.0303819440f);
+  p = __builtin_fmaf(z, p, 0.0446428582f);
+  p = __builtin_fmaf(z, p, 0.0750000030f);
+  p = __builtin_fmaf(z, p, 0.166666672f);
+  return __builtin_fmaf(a * z, p, a);
+}
+__device__ __forceinline__ float asin_c(float x) {
+  const float a = fabsf(x);
+  float r;
+  if (a <= 0.5f) {
+    r = asin_core_c(a);
+  } else {
+    const float s = __builtin_sqrtf((1.0f - a) * 0.5f);
+    r = __builtin_fmaf(-2.0f, asin_core_c(s), 1.57079637f) + -4.37113883e-08f;
+  }
+  return x < 0.0f ? -r : r;
+}
+__device__ __forceinline__ float acos_c(float x) {
+  if (fabsf(x) <= 0.5f) return (1.57079637f - asin_core_c(fabsf(x)) * (x < 0.0f ? -1.0f : 1.0f)) + -4.37113883e-08f;
+  if (x > 0.0f) return 2.0f * asin_core_c(__builtin_sqrtf((1.0f - x) * 0.5f));
+  return __builtin_fmaf(-2.0f, asin_core_c(__builtin_sqrtf((1.0f + x) * 0.5f)), 3.14159274f) + -8.74227766e-08f;
+}
+__device__ __forceinline__ float exp_c(float x) {
+  if (!(x < 89.0f)) return x != x ? x : __builtin_inff();
+  if (x < -104.0f) return 0.0f;
+  const float k = __builtin_rintf(x * 1.44269502f);
+  float r = __builtin_fmaf(-k, 0.693147182f, x);
+  r = __builtin_fmaf(-k, -1.90465421e-09f, r);
+  float p = __builtin_fmaf(r, 2.48015876e-05f, 1.98412701e-04f);
+  p = __builtin_fmaf(r, p, 1.38888892e-03f);
+  p = __builtin_fmaf(r, p, 8.33333377e-03f);
+  p = __builtin_fmaf(r, p, 4.16666679e-02f);
+  p = __builtin_fmaf(r, p, 1.66666672e-01f);
+  p = __builtin_fmaf(r, p, 0.5f);
+  p = __builtin_fmaf(r, p, 1.0f);
+  p = __builtin_fmaf(r, p, 1.0f);
+  return __builtin_ldexpf(p, (int)k);
+}
+__device__ __forceinline__ float log_c(float x) {
+  if (!(x > 0.0f) || x == __builtin_inff()) return x == 0.0f ? -__builtin_inff() : (x > 0.0f ? x : __builtin_nanf(""));
+  int e = 0;
+  if (x < 1.17549435e-38f) {
+    x *= 16777216.0f;
+    e = -24;
+  }
+  unsigned i = __float_as_uint(x);
+  e += (int)((i >> 23) & 0xffu) - 127;
+  float m = __uint_as_float((i & 0x007fffffu) | 0x3f800000u);
+  if (m > 1.41421354f) {
+    m *= 0.5f;
+    e += 1;
+  }
+  const float f = m - 1.0f;
+  const float s = f / (2.0f + f);
+  const float s2 = s * s;
+  float p = __builtin_fmaf(s2, 0.0909090936f, 0.111111112f);
+  p = __builtin_fmaf(s2, p, 0.142857149f);
+  p = __builtin_fmaf(s2, p, 0.200000003f);
+  p = __builtin_fmaf(s2, p, 0.333333343f);
+  const float lm = __builtin_fmaf(2.0f * s * s2, p, 2.0f * s);
+  const float ef = (float)e;
+  return __builtin_fmaf(ef, 0.693147182f, __builtin_fmaf(ef, -1.90465421e-09f, lm));
+}
+__device__ __forceinline__ float pow_c(float x, float y) { return x == 0.0f ? 0.0f : exp_c(y * log_c(x)); }
+
 // One FTL point update, in place (DistanceConstraint, cs_simulation.glsl:110-116):
 //   d = p - lead;  p = lead + (sl * inversesqrt(dot(d, d))) * d
 // with the operations of dot3 / rsqrt_c / fma3 above, in the same order.

@@ -64,16 +64,16 @@ __device__ Cubic solve_cubic_normalized(float a, float b, float c) {
   if (D > 0.0f) {
     const float sqrtD = sqrtf(D);
     const float u = R + sqrtD, v = R - sqrtD;
-    const float s = (u > 0.0f ? 1.0f : (u < 0.0f ? -1.0f : 0.0f)) * powf(fabsf(u), 0.333f);
-    const float t = (v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f)) * powf(fabsf(v), 0.333f);
+    const float s = (u > 0.0f ? 1.0f : (u < 0.0f ? -1.0f : 0.0f)) * ftl_dev::pow_c(fabsf(u), 0.333f);
+    const float t = (v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f)) * ftl_dev::pow_c(fabsf(v), 0.333f);
     r.r[0] = s + t - third_a;
     r.n = 1;
   } else {
-    const float theta = acosf(R * (1.0f / sqrtf(-Q3)));
+    const float theta = ftl_dev::acos_c(R * (1.0f / sqrtf(-Q3)));
     const float two_sqrt_q = 2.0f * sqrtf(-Q);
-    r.r[0] = __builtin_fmaf(two_sqrt_q, cosf(theta / 3.0f), -third_a);
-    r.r[1] = __builtin_fmaf(two_sqrt_q, cosf((theta + 2.0f * kPiRef) / 3.0f), -third_a);
-    r.r[2] = __builtin_fmaf(two_sqrt_q, cosf((theta + 4.0f * kPiRef) / 3.0f), -third_a);
+    r.r[0] = __builtin_fmaf(two_sqrt_q, ftl_dev::cos_c(theta / 3.0f), -third_a);
+    r.r[1] = __builtin_fmaf(two_sqrt_q, ftl_dev::cos_c((theta + 2.0f * kPiRef) / 3.0f), -third_a);
+    r.r[2] = __builtin_fmaf(two_sqrt_q, ftl_dev::cos_c((theta + 4.0f * kPiRef) / 3.0f), -third_a);
     r.n = 3;
   }
   return r;
@@ -106,11 +106,11 @@ __device__ float fresnel(float eta0, float eta_perp, float eta_par, float cosA, 
 __device__ float absorption_term(int p, float absorption, float eta_perp, float eta_par, float cosG, float sinG) {
   if (p == 0) return fresnel(1.0f, eta_perp, eta_par, cosG, sinG);
   const float sinT = sinG / eta_perp;
-  const float gT = asinf(sinT);
-  const float cosT = cosf(gT);
+  const float gT = ftl_dev::asin_c(sinT);
+  const float cosT = ftl_dev::cos_c(gT);
   const float fi = fresnel(1.0f, eta_perp, eta_par, cosG, sinG);
   const float ft = fresnel(1.0f, 1.0f / eta_perp, 1.0f / eta_par, cosT, sinT);
-  const float t = expf(-4.0f * absorption * (cosT * cosT));
+  const float t = ftl_dev::exp_c(-4.0f * absorption * (cosT * cosT));
   const float omf = 1.0f - fi;
   float a = omf * omf;
   for (int k = 0; k < p - 1; ++k) a *= ft;
@@ -129,7 +129,7 @@ __device__ float Np(int p, float absorption, float eta_perp, float eta_par, floa
   float L = 0.0f;
   for (int i = 0; i < roots.n; ++i) {
     const float g = roots.r[i];
-    const float sinG = sinf(g), cosG = cosf(g);
+    const float sinG = ftl_dev::sin_c(g), cosG = ftl_dev::cos_c(g);
     const float a = absorption_term(p, absorption, eta_perp, eta_par, cosG, sinG);
     const float d1 = (3.0f * A * (g * g) + C) / cosG;
     L += a * (1.0f / fabsf(2.0f * d1));
@@ -138,7 +138,7 @@ __device__ float Np(int p, float absorption, float eta_perp, float eta_par, floa
 }
 
 __device__ float gaussian(float sigma, float x_mu) {
-  return expf(-(x_mu * x_mu) / (2.0f * sigma * sigma)) / (2.5066282f * fabsf(sigma));
+  return ftl_dev::exp_c(-(x_mu * x_mu) / (2.0f * sigma * sigma)) / (2.5066282f * fabsf(sigma));
 }
 
 struct LutArgs {
@@ -157,13 +157,13 @@ __global__ __launch_bounds__(256) void ftl_marschner_kernel(LutArgs A) {
     // M: cs_marschner_m.glsl:37-61
     const float sinI = 2.0f * (float)x * A.inv_res - 1.0f;
     const float sinR = 2.0f * (float)y * A.inv_res - 1.0f;
-    const float thI = asinf(sinI), thR = asinf(sinR);
+    const float thI = ftl_dev::asin_c(sinI), thR = ftl_dev::asin_c(sinR);
     const float thH = (thI + thR) / 2.0f, thD = (thI - thR) / 2.0f;
     const float deg = thH * (180.0f / 3.14159265358979323846f);
     const float lr = gaussian(1.0f * A.br, deg - 1.0f * A.ar);
     const float ltt = gaussian(0.5f * A.br, deg - (-0.5f * A.ar));
     const float ltrt = gaussian(2.0f * A.br, deg - (-1.5f * A.ar));
-    A.m[idx] = make_float4(lr, ltt, ltrt, cosf(thD));
+    A.m[idx] = make_float4(lr, ltt, ltrt, ftl_dev::cos_c(thD));
   } else {
     // N: cs_marschner_n.glsl:37-82
     const float cosPhiD = 2.0f * (float)x * A.inv_res - 1.0f;
@@ -172,8 +172,8 @@ __global__ __launch_bounds__(256) void ftl_marschner_kernel(LutArgs A) {
     const float eta2 = A.eta * A.eta;
     const float eta_perp = sqrtf(eta2 - sinThD2) / cosThD;
     const float eta_par = eta2 / eta_perp;
-    const float phiD = acosf(cosPhiD);
-    const float c = asinf(1.0f / eta_perp);
+    const float phiD = ftl_dev::acos_c(cosPhiD);
+    const float c = ftl_dev::asin_c(1.0f / eta_perp);
     const float R = Np(0, A.absorption, eta_perp, eta_par, c, phiD);
     const float TT = Np(1, A.absorption, eta_perp, eta_par, c, phiD);
     const float TRT = Np(2, A.absorption, eta_perp, eta_par, c, phiD);

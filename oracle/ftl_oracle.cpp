@@ -766,6 +766,95 @@ float trig_f32(float x, int cosine) {
 float trig_f32_sin(float x) { return trig_f32(x, 0); }
 float trig_f32_cos(float x) { return trig_f32(x, 1); }
 
+// asin, acos, exp, log and pow of the Marschner tables (cs_marschner_*.glsl),
+// one FP32 definition the device shares, from the mathematics only:
+// * asin on [0, 1/2]: a + a^3 P(a^2), P the Taylor series of asin to a^23
+//   (coefficients (2n)! / (4^n (n!)^2 (2n+1)) rounded to float, Horner with
+//   fma); above 1/2, asin a = pi/2 - 2 asin(sqrt((1 - a)/2)); acos from the same
+//   core (pi/2 - asin, 2 asin(...), pi - 2 asin(...)), pi and pi/2 in two parts;
+// * exp: Cody-Waite by ln 2 in two parts, the Taylor series of e^r to r^8,
+//   then the scaling by 2^k (ldexp, one rounding);
+// * log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with
+//   s = (m - 1)/(m + 1) (series to s^11), plus e ln 2 in two parts;
+// * pow(x, y) = exp(y log x) for x >= 0 (pow(0, y) = 0: y > 0 here).
+// Within 2 ulp (pow: 3 ulp) of float64 on the tables' ranges.
+float asin_core_f32(float a) {
+  const float z = a * a;
+  float p = std::fma(z, 0.00731252600f, 0.00839033537f);
+  p = std::fma(z, p, 0.00976160914f);
+  p = std::fma(z, p, 0.0115518011f);
+  p = std::fma(z, p, 0.0139648439f);
+  p = std::fma(z, p, 0.0173527636f);
+  p = std::fma(z, p, 0.0223721582f);
+  p = std::fma(z, p, 0.0303819440f);
+  p = std::fma(z, p, 0.0446428582f);
+  p = std::fma(z, p, 0.0750000030f);
+  p = std::fma(z, p, 0.166666672f);
+  return std::fma(a * z, p, a);
+}
+float asin_f32(float x) {
+  const float a = std::fabs(x);
+  float r;
+  if (a <= 0.5f) {
+    r = asin_core_f32(a);
+  } else {
+    const float s = std::sqrt((1.0f - a) * 0.5f);
+    r = std::fma(-2.0f, asin_core_f32(s), 1.57079637f) + -4.37113883e-08f;
+  }
+  return x < 0.0f ? -r : r;
+}
+float acos_f32(float x) {
+  if (std::fabs(x) <= 0.5f)
+    return (1.57079637f - asin_core_f32(std::fabs(x)) * (x < 0.0f ? -1.0f : 1.0f)) + -4.37113883e-08f;
+  if (x > 0.0f) return 2.0f * asin_core_f32(std::sqrt((1.0f - x) * 0.5f));
+  return std::fma(-2.0f, asin_core_f32(std::sqrt((1.0f + x) * 0.5f)), 3.14159274f) + -8.74227766e-08f;
+}
+float exp_f32(float x) {
+  if (!(x < 89.0f)) return x != x ? x : INFINITY;
+  if (x < -104.0f) return 0.0f;
+  const float k = std::nearbyint(x * 1.44269502f);
+  float r = std::fma(-k, 0.693147182f, x);
+  r = std::fma(-k, -1.90465421e-09f, r);
+  float p = std::fma(r, 2.48015876e-05f, 1.98412701e-04f);
+  p = std::fma(r, p, 1.38888892e-03f);
+  p = std::fma(r, p, 8.33333377e-03f);
+  p = std::fma(r, p, 4.16666679e-02f);
+  p = std::fma(r, p, 1.66666672e-01f);
+  p = std::fma(r, p, 0.5f);
+  p = std::fma(r, p, 1.0f);
+  p = std::fma(r, p, 1.0f);
+  return std::ldexp(p, static_cast<int>(k));
+}
+float log_f32(float x) {
+  if (!(x > 0.0f) || x == INFINITY) return x == 0.0f ? -INFINITY : (x > 0.0f ? x : NAN);
+  int e = 0;
+  if (x < 1.17549435e-38f) {
+    x *= 16777216.0f;
+    e = -24;
+  }
+  uint32_t i;
+  std::memcpy(&i, &x, 4);
+  e += static_cast<int>((i >> 23) & 0xffu) - 127;
+  i = (i & 0x007fffffu) | 0x3f800000u;
+  float m;
+  std::memcpy(&m, &i, 4);
+  if (m > 1.41421354f) {
+    m *= 0.5f;
+    e += 1;
+  }
+  const float f = m - 1.0f;
+  const float s = f / (2.0f + f);
+  const float s2 = s * s;
+  float p = std::fma(s2, 0.0909090936f, 0.111111112f);
+  p = std::fma(s2, p, 0.142857149f);
+  p = std::fma(s2, p, 0.200000003f);
+  p = std::fma(s2, p, 0.333333343f);
+  const float lm = std::fma(2.0f * s * s2, p, 2.0f * s);
+  const float ef = static_cast<float>(e);
+  return std::fma(ef, 0.693147182f, std::fma(ef, -1.90465421e-09f, lm));
+}
+float pow_f32(float x, float y) { return x == 0.0f ? 0.0f : exp_f32(y * log_f32(x)); }
+
 // glm::simplex(vec2) (glm/gtc/noise.inl; the GLM tree is not part of the
 // reference), from the published algorithm: McEwan, Sheets, Gustavson,
 // Richardson, "Efficient computational noise in GLSL", JGT 17(2), 2012
@@ -833,6 +922,11 @@ void oracle_reference_groom_f32(int nroots, const float* roots, const float* nor
 float oracle_simplex2_f32(float x, float y) { return simplex2_f32(x, y); }
 float oracle_sin_f32(float x) { return trig_f32(x, 0); }
 float oracle_cos_f32(float x) { return trig_f32(x, 1); }
+float oracle_asin_f32(float x) { return asin_f32(x); }
+float oracle_acos_f32(float x) { return acos_f32(x); }
+float oracle_exp_f32(float x) { return exp_f32(x); }
+float oracle_log_f32(float x) { return log_f32(x); }
+float oracle_pow_f32(float x, float y) { return pow_f32(x, y); }
 
 // The tangents of hair.cc:290-328 (curly or not), float4 [nstrands*npoints].
 void oracle_reference_tangents_f32(int nstrands, int npoints, const float* normals, float maxlength, int curly,
@@ -884,6 +978,21 @@ void oracle_reference_tangents_f32(int nstrands, int npoints, const float* norma
 
 namespace {
 
+// The transcendentals of the tables: the shared FP32 definitions above for
+// the FP32 restatement, the C library's for the FP64 accuracy check.
+float m_sin(float x) { return trig_f32(x, 0); }
+float m_cos(float x) { return trig_f32(x, 1); }
+float m_asin(float x) { return asin_f32(x); }
+float m_acos(float x) { return acos_f32(x); }
+float m_exp(float x) { return exp_f32(x); }
+float m_pow(float x, float y) { return pow_f32(x, y); }
+double m_sin(double x) { return std::sin(x); }
+double m_cos(double x) { return std::cos(x); }
+double m_asin(double x) { return std::asin(x); }
+double m_acos(double x) { return std::acos(x); }
+double m_exp(double x) { return std::exp(x); }
+double m_pow(double x, double y) { return std::pow(x, y); }
+
 template <typename T>
 struct MRoots {
   T r[3];
@@ -933,16 +1042,16 @@ MRoots<T> m_cubic(T A, T B, T C, T Dd) {
   if (D > T(0)) {
     const T sd = std::sqrt(D);
     const T u = R + sd, v = R - sd;
-    const T s = (u > T(0) ? T(1) : (u < T(0) ? T(-1) : T(0))) * std::pow(std::fabs(u), T(0.333f));
-    const T t = (v > T(0) ? T(1) : (v < T(0) ? T(-1) : T(0))) * std::pow(std::fabs(v), T(0.333f));
+    const T s = (u > T(0) ? T(1) : (u < T(0) ? T(-1) : T(0))) * m_pow(std::fabs(u), T(0.333f));
+    const T t = (v > T(0) ? T(1) : (v < T(0) ? T(-1) : T(0))) * m_pow(std::fabs(v), T(0.333f));
     r.r[0] = s + t - third;
     r.n = 1;
   } else {
-    const T theta = std::acos(R * (T(1) / std::sqrt(-Q3)));
+    const T theta = m_acos(R * (T(1) / std::sqrt(-Q3)));
     const T tq = T(2) * std::sqrt(-Q);
-    r.r[0] = std::fma(tq, std::cos(theta / T(3)), -third);
-    r.r[1] = std::fma(tq, std::cos((theta + T(2) * pi) / T(3)), -third);
-    r.r[2] = std::fma(tq, std::cos((theta + T(4) * pi) / T(3)), -third);
+    r.r[0] = std::fma(tq, m_cos(theta / T(3)), -third);
+    r.r[1] = std::fma(tq, m_cos((theta + T(2) * pi) / T(3)), -third);
+    r.r[2] = std::fma(tq, m_cos((theta + T(4) * pi) / T(3)), -third);
     r.n = 3;
   }
   return r;
@@ -968,10 +1077,10 @@ template <typename T>
 T m_absorption(int p, T absorption, T eperp, T epar, T cosG, T sinG) {
   if (p == 0) return m_fresnel(T(1), eperp, epar, cosG, sinG);
   const T sinT = sinG / eperp;
-  const T cosT = std::cos(std::asin(sinT));
+  const T cosT = m_cos(m_asin(sinT));
   const T fi = m_fresnel(T(1), eperp, epar, cosG, sinG);
   const T ft = m_fresnel(T(1), T(1) / eperp, T(1) / epar, cosT, sinT);
-  const T t = std::exp(T(-4) * absorption * (cosT * cosT));
+  const T t = m_exp(T(-4) * absorption * (cosT * cosT));
   T a = (T(1) - fi) * (T(1) - fi);
   for (int k = 0; k < p - 1; ++k) a *= ft;
   for (int k = 0; k < p; ++k) a *= t;
@@ -987,8 +1096,8 @@ T m_np(int p, T absorption, T eperp, T epar, T c, T phi) {
   T L = T(0);
   for (int i = 0; i < roots.n; ++i) {
     const T g = roots.r[i];
-    const T a = m_absorption(p, absorption, eperp, epar, std::cos(g), std::sin(g));
-    const T d1 = (T(3) * A * (g * g) + C) / std::cos(g);
+    const T a = m_absorption(p, absorption, eperp, epar, m_cos(g), m_sin(g));
+    const T d1 = (T(3) * A * (g * g) + C) / m_cos(g);
     L += a * (T(1) / std::fabs(T(2) * d1));
   }
   return std::min(L, T(1));
@@ -996,7 +1105,7 @@ T m_np(int p, T absorption, T eperp, T epar, T c, T phi) {
 
 template <typename T>
 T m_gauss(T sigma, T xmu) {
-  return std::exp(-(xmu * xmu) / (T(2) * sigma * sigma)) / (T(2.5066282f) * std::fabs(sigma));
+  return m_exp(-(xmu * xmu) / (T(2) * sigma * sigma)) / (T(2.5066282f) * std::fabs(sigma));
 }
 
 template <typename T>
@@ -1007,21 +1116,21 @@ void m_luts(int res, float eta_f, float absorption_f, float ar_f, float br_f, T*
     for (int x = 0; x < res; ++x) {
       T* m = M + 4 * ((size_t)y * res + x);
       const T sI = T(2) * T(x) * inv - T(1), sR = T(2) * T(y) * inv - T(1);
-      const T tI = std::asin(sI), tR = std::asin(sR);
+      const T tI = m_asin(sI), tR = m_asin(sR);
       const T tH = (tI + tR) / T(2), tD = (tI - tR) / T(2);
       const T deg = tH * (T(180.0f) / T(3.14159265358979323846f));
       m[0] = m_gauss(T(1) * br, deg - T(1) * ar);
       m[1] = m_gauss(T(0.5f) * br, deg - T(-0.5f) * ar);
       m[2] = m_gauss(T(2) * br, deg - T(-1.5f) * ar);
-      m[3] = std::cos(tD);
+      m[3] = m_cos(tD);
       T* n = N + 4 * ((size_t)y * res + x);
       const T cPhi = T(2) * T(x) * inv - T(1), cTh = T(2) * T(y) * inv - T(1);
       const T s2 = T(1) - cTh * cTh;
       const T eta2 = eta * eta;
       const T eperp = std::sqrt(eta2 - s2) / cTh;
       const T epar = eta2 / eperp;
-      const T phi = std::acos(cPhi);
-      const T c = std::asin(T(1) / eperp);
+      const T phi = m_acos(cPhi);
+      const T c = m_asin(T(1) / eperp);
       n[0] = m_np(0, absorption, eperp, epar, c, phi);
       n[1] = m_np(1, absorption, eperp, epar, c, phi);
       n[2] = m_np(2, absorption, eperp, epar, c, phi);

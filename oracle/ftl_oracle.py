@@ -71,9 +71,12 @@ def lib():
         L.oracle_reference_groom_f32.argtypes = [C.c_int, vp, vp, C.c_int, C.c_float, C.c_uint64, vp, vp]
         L.oracle_simplex2_f32.restype = C.c_float
         L.oracle_simplex2_f32.argtypes = [C.c_float, C.c_float]
-        for fn in (L.oracle_sin_f32, L.oracle_cos_f32):
+        for fn in (L.oracle_sin_f32, L.oracle_cos_f32, L.oracle_asin_f32, L.oracle_acos_f32, L.oracle_exp_f32,
+                   L.oracle_log_f32):
             fn.restype = C.c_float
             fn.argtypes = [C.c_float]
+        L.oracle_pow_f32.restype = C.c_float
+        L.oracle_pow_f32.argtypes = [C.c_float, C.c_float]
         L.oracle_reference_tangents_f32.argtypes = [C.c_int, C.c_int, vp, C.c_float, C.c_int, vp]
         L.oracle_marschner_f32.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, vp, vp]
         L.oracle_marschner_f64.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, vp, vp]
@@ -293,6 +296,12 @@ def sin_f32(x):
 
 def cos_f32(x):
     return float(lib().oracle_cos_f32(f32(x)))
+
+
+def math_f32(name, *args):
+    """One of the shared FP32 transcendentals of the Marschner tables:
+    'asin', 'acos', 'exp', 'log' or 'pow' (two arguments)."""
+    return float(getattr(lib(), "oracle_%s_f32" % name)(*[f32(a) for a in args]))
 
 
 def reference_tangents_f32(normals, npoints, maxlength, curly=True):
