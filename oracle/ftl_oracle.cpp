@@ -37,9 +37,12 @@
 // same IEEE operations in the same order (the HIP kernels do) produces the same
 // bits.  `inversesqrt` is left to the implementation by GLSL (2 ulp); the FP32
 // path uses a bit-reproducible integer-seeded Newton iteration (<= 2 ulp,
-// checked exhaustively over one binade pair by the tests).  The FP64 path
-// evaluates the same formulas in double with 1/sqrt, and serves as the
-// accuracy reference for the FP32 path.
+// checked exhaustively over one binade pair by the tests).  sin, cos, asin,
+// acos, exp, log and pow (the tangents and the Marschner tables) are likewise
+// one FP32 definition each, built from their series (trig_f32, asin_f32, ...;
+// within 2-3 ulp, checked against float64 by the tests).  The FP64 path
+// evaluates the same formulas in double with 1/sqrt and the C library's
+// functions, and serves as the accuracy reference for the FP32 path.
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off, no fast-math).
 
