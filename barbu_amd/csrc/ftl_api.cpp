@@ -581,7 +581,8 @@ static ftl_status groom_step(ftl_groom* g, float dt, void* stream, hipEvent_t st
   if (cls < 0) return ftl_fail(FTL_ERR_UNSUPPORTED, "no launch class for npoints=%d", g->npoints);
   hipError_t e;
   if ((e = hipSetDevice(g->ctx->device)) != hipSuccess) return ftl_hip_fail(e, "hipSetDevice");
-  if ((e = ftl_launch_plain(cls, A, (hipStream_t)stream, start, stop)) != hipSuccess)
+  const bool nt = (size_t)g->nstrands * g->npoints * 2 * sizeof(float4) > ftl_stream_store_bytes();
+  if ((e = ftl_launch_plain(cls, A, (hipStream_t)stream, start, stop, nt)) != hipSuccess)
     return ftl_hip_fail(e, "FTL step launch");
   return FTL_OK;
 }

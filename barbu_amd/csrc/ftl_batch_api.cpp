@@ -70,7 +70,7 @@ ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::ve
   std::vector<GroomPlan> plan;
   plan.reserve(n);
   std::vector<double> work(FTL_NCLASSES, 0.0);
-  size_t total_tiles = 0;
+  size_t total_tiles = 0, state_bytes = 0;
   for (int i = 0; i < n; ++i) {
     ftl_groom* g = grooms[i];
     if (!g) return ftl_fail(FTL_ERR_INVALID_ARGUMENT, "groom %d is NULL", i);
@@ -85,7 +85,10 @@ ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::ve
     plan.push_back(GroomPlan{i, cls, cost, nt, ns_tile});
     work[cls] += ns_tile * nt;
     total_tiles += (size_t)nt;
+    state_bytes += (size_t)g->nstrands * g->npoints * 2 * sizeof(float4);
   }
+  // the class kernels of a batch run together: the batch's whole state decides
+  const bool stream_stores = state_bytes > ftl_stream_store_bytes();
   if (total_tiles > (size_t)INT32_MAX) return ftl_fail(FTL_ERR_UNSUPPORTED, "too many tiles in one batch");
   {
     std::vector<ftl_groom*> seen(grooms, grooms + n);
@@ -116,7 +119,7 @@ ftl_status ftl_plan_tiles(ftl_ctx* ctx, ftl_groom* const* grooms, int n, std::ve
       ntiles += gp.ntiles;
     }
     const int count = (int)spans.size() - first;
-    if (count > 0) runs.push_back(ClassRun{cls, first, count, ntiles, ntiles});
+    if (count > 0) runs.push_back(ClassRun{cls, first, count, ntiles, ntiles, stream_stores});
   }
   if (runs.size() > 64) return ftl_fail(FTL_ERR_UNSUPPORTED, "too many launch classes");
   return FTL_OK;
@@ -135,7 +138,8 @@ hipError_t ftl_enqueue_runs(const std::vector<ClassRun>& runs, const FtlStep* d_
     hipStream_t rs = (par && r > 0) ? (*side)[r - 1] : s;
     if (par && r > 0 && (e = hipStreamWaitEvent(rs, fork, 0)) != hipSuccess) return e;
     const ClassRun& cr = runs[r];
-    if ((e = ftl_launch_tiles(cr.cls, d_steps, d_spans + cr.first, cr.count, cr.ntiles, cr.grid, rs)) != hipSuccess)
+    if ((e = ftl_launch_tiles(cr.cls, d_steps, d_spans + cr.first, cr.count, cr.ntiles, cr.grid, rs, cr.nt)) !=
+        hipSuccess)
       return e;
     if (par && r > 0) {
       if ((e = hipEventRecord((*side_ev)[r - 1], rs)) != hipSuccess) return e;

@@ -128,6 +128,7 @@ int ftl_default_class(int npoints);
 // span list, ntiles tiles in all, stepped by grid workgroups.
 struct ClassRun {
   int cls, first, count, ntiles, grid;
+  bool nt;  // non-temporal stores (the batch's state exceeds ftl_stream_store_bytes())
 };
 // Groups the grooms by class (one span per groom, in tile order within its
 // class) and sizes the class grids.

@@ -164,8 +164,8 @@ __global__ __launch_bounds__(256) void ftl_upload_kernel(const uint4* __restrict
 // Host dispatch.
 
 #define FTL_DECL(id, M, G, GU, W, CH)                                                                    \
-  hipError_t ftl_launch_plain_##id(const FtlStep& A, hipStream_t stream, hipEvent_t, hipEvent_t);      \
-  hipError_t ftl_launch_tiles_##id(const FtlStep*, const FtlSpan*, int, int, int, hipStream_t);     \
+  hipError_t ftl_launch_plain_##id(const FtlStep& A, hipStream_t stream, hipEvent_t, hipEvent_t, bool); \
+  hipError_t ftl_launch_tiles_##id(const FtlStep*, const FtlSpan*, int, int, int, hipStream_t, bool);    \
   int ftl_tiles_occ_##id();
 FTL_CLASSES(FTL_DECL)
 #undef FTL_DECL
@@ -186,11 +186,14 @@ int ftl_class_m(int cls) { return (cls >= 0 && cls < FTL_NCLASSES) ? kFtlClasses
 int ftl_class_g(int cls) { return (cls >= 0 && cls < FTL_NCLASSES) ? kFtlClasses[cls].g : 0; }
 int ftl_class_guarded(int cls) { return (cls >= 0 && cls < FTL_NCLASSES) ? kFtlClasses[cls].guarded : 0; }
 
-hipError_t ftl_launch_plain(int cls, const FtlStep& A, hipStream_t stream, hipEvent_t start, hipEvent_t stop) {
+size_t ftl_stream_store_bytes() { return (size_t)384 << 20; }
+
+hipError_t ftl_launch_plain(int cls, const FtlStep& A, hipStream_t stream, hipEvent_t start, hipEvent_t stop,
+                            bool nt) {
   switch (cls) {
 #define FTL_LP(id, M, G, GU, W, CH) \
   case id:                   \
-    return ftl_launch_plain_##id(A, stream, start, stop);
+    return ftl_launch_plain_##id(A, stream, start, stop, nt);
     FTL_CLASSES(FTL_LP)
 #undef FTL_LP
   }
@@ -198,11 +201,11 @@ hipError_t ftl_launch_plain(int cls, const FtlStep& A, hipStream_t stream, hipEv
 }
 
 hipError_t ftl_launch_tiles(int cls, const FtlStep* grooms, const FtlSpan* spans, int nspans, int ntiles, int grid,
-                            hipStream_t stream) {
+                            hipStream_t stream, bool nt) {
   switch (cls) {
 #define FTL_LB(id, M, G, GU, W, CH) \
   case id:                   \
-    return ftl_launch_tiles_##id(grooms, spans, nspans, ntiles, grid, stream);
+    return ftl_launch_tiles_##id(grooms, spans, nspans, ntiles, grid, stream, nt);
     FTL_CLASSES(FTL_LB)
 #undef FTL_LB
   }

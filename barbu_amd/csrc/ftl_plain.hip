@@ -1,7 +1,9 @@
 // Plain (one groom) FTL step kernels for one launch class, selected at build
 // time with -DFTL_CLASS_ID=<id>: one wave64 workgroup per tile of strands, or,
 // for launches with wind, workgroups of four tiles sharing an LDS copy of the
-// wind lattice; and the tile-list kernel of batched frames.
+// wind lattice; and the tile-list kernel of batched frames.  Each kernel comes
+// in two forms, NT (non-temporal stores, for launches whose state exceeds
+// ftl_stream_store_bytes()) and not.
 #include <hip/hip_ext.h>
 
 #include "ftl_classes.h"
@@ -40,6 +42,7 @@ constexpr int kWavesPerEu = CL::W;
 // waves per SIMD (their register budget), FTL_LOAD_AHEAD for the others.
 constexpr int kLoadAhead = kWavesPerEu >= 6 ? 1 : FTL_LOAD_AHEAD;
 
+template <bool NT>
 __global__ __launch_bounds__(FTL_WAVE) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
 void ftl_plain_kernel(FtlStep arg) {
   (void)arg;
@@ -52,7 +55,7 @@ void ftl_plain_kernel(FtlStep arg) {
   const FtlStep& A = arg;
 #endif
   __shared__ ftl_tile_impl::TileLds<CL::M, CL::G, CL::GU> sh;
-  ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead>(A, blockIdx.x * (FTL_WAVE / CL::G), sh);
+  ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, NT, kLoadAhead>(A, blockIdx.x * (FTL_WAVE / CL::G), sh);
 }
 
 // With wind: four tiles (waves) per workgroup share one LDS copy of the wind
@@ -78,6 +81,7 @@ constexpr size_t kWindGroupLds =
 constexpr int kWindGroupsPerCu = (kWavesPerEu == 2 ? 8 : 16) / kWindWaves;
 constexpr bool kWindLds = (kWavesPerEu == 4 || kWavesPerEu == 2) && kWindGroupsPerCu * kWindGroupLds <= 160 * 1024;
 
+template <bool NT>
 __global__ __launch_bounds__(FTL_WAVE * kWindWaves) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
 void ftl_plain_wind_kernel(FtlStep arg) {
   (void)arg;
@@ -98,7 +102,7 @@ void ftl_plain_wind_kernel(FtlStep arg) {
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / FTL_WAVE);
     const int t = (int)blockIdx.x * kWindWaves + w;
     if (t < ntiles)
-      ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead, kWindHalf ? 2 : 1>(A, t * spt, sh[w], lattice);
+      ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, NT, kLoadAhead, kWindHalf ? 2 : 1>(A, t * spt, sh[w], lattice);
   }
 }
 
@@ -110,16 +114,22 @@ void ftl_plain_wind_kernel(FtlStep arg) {
 // start / stop (may be null): events that take the kernel's own start and end
 // timestamps (hipExtLaunchKernelGGL), for per-launch timing.
 hipError_t FTL_PASTE(ftl_launch_plain_, FTL_CLASS_ID)(const FtlStep& A, hipStream_t stream, hipEvent_t start,
-                                                      hipEvent_t stop) {
+                                                      hipEvent_t stop, bool nt) {
   constexpr int spt = FTL_WAVE / CL::G;
   const int nblocks = (A.nstrands + spt - 1) / spt;
   if (nblocks <= 0) return hipSuccess;
   if (kWindLds && FTL_WIND_LDS && A.wind_grid) {
-    hipExtLaunchKernelGGL(ftl_plain_wind_kernel, dim3((nblocks + kWindWaves - 1) / kWindWaves),
-                          dim3(FTL_WAVE * kWindWaves), 0, stream, start, stop, 0, A);
+    const dim3 grid((nblocks + kWindWaves - 1) / kWindWaves), block(FTL_WAVE * kWindWaves);
+    if (nt)
+      hipExtLaunchKernelGGL(ftl_plain_wind_kernel<true>, grid, block, 0, stream, start, stop, 0, A);
+    else
+      hipExtLaunchKernelGGL(ftl_plain_wind_kernel<false>, grid, block, 0, stream, start, stop, 0, A);
     return hipGetLastError();
   }
-  hipExtLaunchKernelGGL(ftl_plain_kernel, dim3(nblocks), dim3(FTL_WAVE), 0, stream, start, stop, 0, A);
+  if (nt)
+    hipExtLaunchKernelGGL(ftl_plain_kernel<true>, dim3(nblocks), dim3(FTL_WAVE), 0, stream, start, stop, 0, A);
+  else
+    hipExtLaunchKernelGGL(ftl_plain_kernel<false>, dim3(nblocks), dim3(FTL_WAVE), 0, stream, start, stop, 0, A);
   return hipGetLastError();
 }
 
@@ -136,6 +146,7 @@ hipError_t FTL_PASTE(ftl_launch_plain_, FTL_CLASS_ID)(const FtlStep& A, hipStrea
 
 namespace {
 
+template <bool NT>
 __global__ __launch_bounds__(FTL_WAVE) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, 8)))
 void ftl_tiles_kernel(const FtlStep* __restrict__ grooms, const FtlSpan* __restrict__ spans, int nspans,
                       int ntiles) {
@@ -153,22 +164,25 @@ void ftl_tiles_kernel(const FtlStep* __restrict__ grooms, const FtlSpan* __restr
     }
     const int gi = __builtin_amdgcn_readfirstlane(spans[lo].groom);
     const int s0 = (t - __builtin_amdgcn_readfirstlane(spans[lo].tile0)) * spt;
-    ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, kLoadAhead>(grooms[gi], s0, sh);
+    ftl_tile_impl::ftl_tile<CL::M, CL::G, CL::GU, NT, kLoadAhead>(grooms[gi], s0, sh);
   }
 }
 
 }  // namespace
 
 hipError_t FTL_PASTE(ftl_launch_tiles_, FTL_CLASS_ID)(const FtlStep* grooms, const FtlSpan* spans, int nspans,
-                                                      int ntiles, int grid, hipStream_t stream) {
+                                                      int ntiles, int grid, hipStream_t stream, bool nt) {
   if (ntiles <= 0 || nspans <= 0) return hipSuccess;
-  hipLaunchKernelGGL(ftl_tiles_kernel, dim3(grid), dim3(FTL_WAVE), 0, stream, grooms, spans, nspans, ntiles);
+  if (nt)
+    hipLaunchKernelGGL(ftl_tiles_kernel<true>, dim3(grid), dim3(FTL_WAVE), 0, stream, grooms, spans, nspans, ntiles);
+  else
+    hipLaunchKernelGGL(ftl_tiles_kernel<false>, dim3(grid), dim3(FTL_WAVE), 0, stream, grooms, spans, nspans, ntiles);
   return hipGetLastError();
 }
 
 int FTL_PASTE(ftl_tiles_occ_, FTL_CLASS_ID)() {
   int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ftl_tiles_kernel, FTL_WAVE, 0) != hipSuccess) {
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ftl_tiles_kernel<false>, FTL_WAVE, 0) != hipSuccess) {
     (void)hipGetLastError();
     return 0;
   }

@@ -155,8 +155,11 @@ __device__ __forceinline__ void tile_sync() {
 // Writes chunk c of the tile (C points per lane, staged in LDS) to dst in
 // the coalesced order of the load.  Called by the whole wave.  gsel >= 0
 // writes only the points owned by lanes with g == gsel (the lanes of a
-// strand finish at different time steps in the pipelined forms).
-template <int M, int G, bool GUARD>
+// strand finish at different time steps in the pipelined forms).  NT: the
+// stores carry the non-temporal hint (launches whose state is too large for
+// the next step to find any of it in the caches).
+typedef float ftl_nt_f4 __attribute__((ext_vector_type(4)));
+template <int M, int G, bool GUARD, bool NT>
 __device__ __forceinline__ void flush_chunk(float4* __restrict__ dst, int strand0, int nstr, int N, int c,
                                             const float4* __restrict__ stage, int lane, int gsel = -1) {
   constexpr int C = Tile<M, G, GUARD>::C;
@@ -179,7 +182,13 @@ __device__ __forceinline__ void flush_chunk(float4* __restrict__ dst, int strand
   tile_sync();
 #pragma unroll
   for (int k = 0; k < C; ++k)
-    if (mine && s0 + k * RS < nstr) d0[(size_t)(k * RS) * N] = t0[k * FTL_WAVE];
+    if (mine && s0 + k * RS < nstr) {
+      const float4 t = t0[k * FTL_WAVE];
+      if constexpr (NT)
+        __builtin_nontemporal_store(ftl_nt_f4{t.x, t.y, t.z, t.w}, (ftl_nt_f4*)&d0[(size_t)(k * RS) * N]);
+      else
+        d0[(size_t)(k * RS) * N] = t;
+    }
   tile_sync();
 }
 
@@ -242,7 +251,7 @@ __device__ __forceinline__ void collide_sweep(const Spheres& S, float3 (&p)[M], 
 // also forms the velocities, stages them and flushes every completed chunk;
 // the lane's last point is left to the caller (its velocity is the raw
 // correction and its collision comes after the sweep).
-template <int M, bool GUARD, bool FINAL>
+template <int M, bool GUARD, bool FINAL, bool NT>
 __device__ __forceinline__ void sweep(const FtlStep& A, const VelArgs& V, const Spheres& S, float3 (&p)[M],
                                       const LenLds& l, int nl,
                                       const float4* __restrict__ pp, float3& last_pre, float3& clast,
@@ -278,7 +287,7 @@ __device__ __forceinline__ void sweep(const FtlStep& A, const VelArgs& V, const 
         p[m] = pm;
         const float3 v = out_velocity(V, pp, m, pm, u);
         stage[slot<C>(lane, m % C)] = make_float4(v.x, v.y, v.z, 0.0f);
-        if (m % C == C - 1) flush_chunk<M, 1, GUARD>(A.vel, strand0, A.nstrands, N, m / C, stage, lane);
+        if (m % C == C - 1) flush_chunk<M, 1, GUARD, NT>(A.vel, strand0, A.nstrands, N, m / C, stage, lane);
       }
       if (GUARD ? (j == nl - 1) : (j == M - 1)) {
         clast = c;
@@ -314,7 +323,7 @@ __device__ __forceinline__ float from_prev_lane(float x) {
 // swept (finite, never stored) but take no velocity work, and the tip itself
 // is finished at its own slot in the last iteration (its velocity is its own
 // correction, as for the last lane of a full strand).
-template <int M, int G, bool FINS, bool GUARD = false>
+template <int M, int G, bool FINS, bool GUARD, bool NT>
 __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, const Spheres& S, float3 (&p)[M],
                                            const LenRegs<M>& l,
                                            float3 lead, int g, bool active, bool fin, int gfin,
@@ -360,7 +369,7 @@ __device__ __forceinline__ void sweep_pipe(const FtlStep& A, const VelArgs& V, c
             stage[slot<C>(lane, m % C)] = make_float4(v.x, v.y, v.z, 0.0f);
           }
           if (m % C == C - 1 && m / C < M / C - 1)
-            flush_chunk<M, G, GUARD>(A.vel, strand0, A.nstrands, NS, m / C, stage, lane, gfin);
+            flush_chunk<M, G, GUARD, NT>(A.vel, strand0, A.nstrands, NS, m / C, stage, lane, gfin);
         } else if (fin && real && strand0 + lane / G < A.nstrands) {
           const float3 v = out_velocity(Vv, pp, m, pm, u);
           A.vel[(size_t)(strand0 + lane / G) * NS + g * M + m] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -419,7 +428,7 @@ struct TileLds : TileLens<M, G == 1> {
 // LW: where the wind gathers read the groom's wind lattice: 0 global memory,
 // 1 wl = the whole lattice copied to LDS by the workgroup, 2 wl = its half
 // lattice (first float4 of each node) in LDS.
-template <int M, int G, bool GUARD, int PD_ = FTL_LOAD_AHEAD, int LW = 0>
+template <int M, int G, bool GUARD, bool NT, int PD_ = FTL_LOAD_AHEAD, int LW = 0>
 __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<M, G, GUARD>& sh,
                                          const float4* wl = nullptr) {
   float4* __restrict__ stage = sh.stage;
@@ -584,8 +593,8 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
   bool bad = false;
   if constexpr (G == 1) {
     for (int it = 0; it < K - 1; ++it)
-      sweep<M, GUARD, false>(A, V, S, p, l, nl, pp, last_pre, clast, stage, strand0, N, lane, cm);
-    sweep<M, GUARD, true>(A, V, S, p, l, nl, pp, last_pre, clast, stage, strand0, N, lane, cm);
+      sweep<M, GUARD, false, NT>(A, V, S, p, l, nl, pp, last_pre, clast, stage, strand0, N, lane, cm);
+    sweep<M, GUARD, true, NT>(A, V, S, p, l, nl, pp, last_pre, clast, stage, strand0, N, lane, cm);
     // The last point: velocity = raw correction, collisions deferred to here.
     u = clast;
 #pragma unroll
@@ -598,7 +607,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
         bad = !(isfinite(pm.x) && isfinite(pm.y) && isfinite(pm.z) && isfinite(v.x) && isfinite(v.y) &&
                 isfinite(v.z));
         stage[slot<C>(lane, j % C)] = make_float4(v.x, v.y, v.z, 0.0f);
-        flush_chunk<M, G, GUARD>(A.vel, strand0, nstr, N, j / C, stage, lane);
+        flush_chunk<M, G, GUARD, NT>(A.vel, strand0, nstr, N, j / C, stage, lane);
       }
     }
   } else {
@@ -612,14 +621,14 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
       const float3 li = f3(from_prev_lane(last_pre.x), from_prev_lane(last_pre.y), from_prev_lane(last_pre.z));
       // (lane 0 of a strand never reads its lead: its slot 0 is the root)
       const float3 lead = li;
-      sweep_pipe<M, G, false, GUARD>(A, V, S, p, l, lead, g, t >= g, false, -1, pp, stage, strand0, lane, u0,
+      sweep_pipe<M, G, false, GUARD, NT>(A, V, S, p, l, lead, g, t >= g, false, -1, pp, stage, strand0, lane, u0,
                                      last_pre, cfirst, clast, cm, N, tip, &tbad);
     }
     for (int t = K - 1; t < K + G - 1; ++t) {
       const float3 li = f3(from_prev_lane(last_pre.x), from_prev_lane(last_pre.y), from_prev_lane(last_pre.z));
       const float3 lead = li;
       const int it = t - g;
-      sweep_pipe<M, G, true, GUARD>(A, V, S, p, l, lead, g, it >= 0 && it < K, it == K - 1, t - (K - 1), pp,
+      sweep_pipe<M, G, true, GUARD, NT>(A, V, S, p, l, lead, g, it >= 0 && it < K, it == K - 1, t - (K - 1), pp,
                                     stage, strand0, lane, u0, last_pre, cfirst, clast, cm, N, tip, &tbad);
     }
     // The lane's last point: its velocity needs the next lane's correction.
@@ -638,7 +647,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
     const float3 v = out_velocity(V, pp, own ? M - 1 : 0, pm, u);
     if constexpr (kStagedVel<M, G>) {
       if (own) stage[slot<C>(lane, (M - 1) % C)] = make_float4(v.x, v.y, v.z, 0.0f);
-      flush_chunk<M, G, GUARD>(A.vel, strand0, nstr, N, M / C - 1, stage, lane);
+      flush_chunk<M, G, GUARD, NT>(A.vel, strand0, nstr, N, M / C - 1, stage, lane);
     } else if (valid && own) {
       A.vel[(size_t)strand * N + g * M + (M - 1)] = make_float4(v.x, v.y, v.z, 0.0f);
     }
@@ -665,7 +674,7 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
       const float3 q = p[c * C + j];
       stage[slot<C>(lane, j)] = make_float4(q.x, q.y, q.z, w[j]);
     }
-    flush_chunk<M, G, GUARD>(A.pos, strand0, nstr, N, c, stage, lane);
+    flush_chunk<M, G, GUARD, NT>(A.pos, strand0, nstr, N, c, stage, lane);
   }
   bad = bad && valid;
   const unsigned long long b = __ballot(bad);

@@ -27,7 +27,14 @@ COPY_KERNEL = "ftl_copy_kernel"
 
 def short(name):
     n = name.replace("(anonymous namespace)::", "")
+    if n.startswith("void "):
+        n = n[5:]
     return n.split("(")[0]
+
+
+def base(name):
+    """Kernel name without template arguments (ftl_tiles_kernel<true> -> ftl_tiles_kernel)."""
+    return name.split("<")[0]
 
 
 def trace(db):
@@ -147,7 +154,7 @@ def traffic_ids(fetch_db, write_db, copy_bytes, kernels):
            "fetch x%.4f, write x%.4f" % (cf / 1024, cw / 1024, copy_bytes, copy_bytes, kf, kw)]
     tr = tw = 0.0
     for k in sorted(set(f) & set(w)):
-        if k[0] not in kernels:
+        if k[0] not in kernels and base(k[0]) not in kernels:
             continue
         rb = f[k][1]["FETCH_SIZE"] * 1024.0 * kf
         wb = w[k][1]["WRITE_SIZE"] * 1024.0 * kw
@@ -193,7 +200,10 @@ def main(argv):
         print(text)
         if len(argv) >= 7:
             path, workload, kernel = argv[4], argv[5], argv[6]
-            rb, wb = res[kernel]
+            hits = [v for k, v in res.items() if k == kernel or base(k) == kernel]
+            if len(hits) != 1:
+                raise SystemExit("%d kernels named %s in the runs" % (len(hits), kernel))
+            rb, wb = hits[0]
             store_traffic(path, workload, kernel, rb, wb)
     else:
         print(__doc__)
