@@ -332,7 +332,7 @@ class Batch:
 
 
 def batch_step(ctx, grooms, dt, stream=None):
-    """One persistent launch over grooms (ftl_batch_step)."""
+    """One batched step over grooms (ftl_batch_step): a tile-list kernel per launch class."""
     arr = (C.c_void_p * len(grooms))(*[g.handle.value for g in grooms])
     dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (len(grooms),)))
     check(lib().ftl_batch_step(ctx.handle, arr, len(grooms), _ptr(dt), stream), "ftl_batch_step")

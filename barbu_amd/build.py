@@ -3,8 +3,8 @@
     python -m barbu_amd.build           # or barbu_amd.build.build()
 
 Every launch class is its own translation unit (ftl_plain.hip with
--DFTL_CLASS_ID=k: the plain kernel and the persistent batched kernel of that
-class), so the objects compile in parallel.
+-DFTL_CLASS_ID=k: the plain kernels and the tile-list kernel of batched steps
+of that class), so the objects compile in parallel.
 """
 import concurrent.futures
 import os

@@ -6,7 +6,7 @@
 
 // X(id, points per lane, lanes per strand, guarded, waves per SIMD targeted
 // by the register allocator: the highest at which neither the plain nor the
-// persistent kernel of the class spills, points per lane of a load/store
+// tile-list kernel of the class spills, points per lane of a load/store
 // chunk: 8 float4 make one 128-byte line, so each line of the groom is read
 // by one request; 4 halves the LDS stage where a bigger one would cost
 // occupancy, at the price of two requests per line)

@@ -123,7 +123,6 @@ int ftl_default_class(int npoints);
 
 // Batched steps (ftl_batch_api.cpp), shared by the batch objects and the
 // serving loop.
-// One persistent launch: the tiles [first, first + count) of one class.
 // One class's launch in a batch: spans [first, first + count) of the batch's
 // span list, ntiles tiles in all, stepped by grid workgroups.
 struct ClassRun {

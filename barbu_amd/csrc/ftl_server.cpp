@@ -8,9 +8,9 @@
 //   2. launches: while a slot is free and requests of idle resident
 //      creatures are pending, it takes them in arrival order (one per
 //      creature, up to the per-batch traffic cap), applies each request's
-//      inputs to its groom and launches one batched persistent step on the
-//      slot's streams (tile plan -> pinned descriptors -> async copies ->
-//      one tile-list kernel per launch class, classes side by side).
+//      inputs to its groom and launches one batched step on the slot's
+//      streams (tile plan -> pinned descriptors -> upload kernel -> one
+//      tile-list kernel per launch class, classes side by side).
 // Slots overlap on the device, so requests that arrive while a batch runs
 // start as soon as a slot frees, not when the device drains.
 #include <hip/hip_runtime.h>

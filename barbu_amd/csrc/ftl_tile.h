@@ -433,8 +433,8 @@ __device__ __forceinline__ void ftl_tile(const FtlStep& A, int strand0, TileLds<
                                          const float4* wl = nullptr) {
   float4* __restrict__ stage = sh.stage;
   constexpr int C = Tile<M, G, GUARD>::C;
-  // Opaque: in the persistent kernels nothing lane-derived is hoisted out of
-  // the tile loop (it would stay live, in registers, across every tile).
+  // Opaque: in the tile-list kernel (a workgroup may step several tiles)
+  // nothing lane-derived is hoisted out of the tile loop (it would stay live, in registers, across every tile).
   int tid = (int)threadIdx.x;
   asm volatile("" : "+v"(tid));
   const int lane = tid & (FTL_WAVE - 1);

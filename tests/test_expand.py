@@ -185,10 +185,10 @@ def test_generators():
 @pytest.mark.gpu
 @pytest.mark.parametrize("S,N,T,ninst,nlines,nsub,scale", [
     (9, 5, 7, 3, 2, 16, 1.0), (40, 2, 70, 1, 1, 1, 0.7), (33, 4, 65, 2, 3, 5, 1.3), (300, 32, 500, 3, 2, 16, 1.0),
-    # the reference's 3 x 2 x 16 over more batches of 8 patches than resident
-    # workgroups (each steps several, loading the next during its stores), the
-    # last batch partial
-    (2000, 4, 30001, 3, 2, 16, 1.0)])
+    # the reference's 3 x 2 x 16 (batches of four patches): 90,003 patches, the
+    # last batch holding 3; one patch (a single partial batch); 6 patches (the
+    # second batch holding 2)
+    (2000, 4, 30001, 3, 2, 16, 1.0), (3, 2, 1, 3, 2, 16, 1.0), (5, 3, 3, 3, 2, 16, 0.5)])
 def test_gpu_matches_oracle(ctx, S, N, T, ninst, nlines, nsub, scale):
     """The pass against the oracle, bit for bit: the batched kernel at the
     reference's tessellation (3 instances x 2 lines x 16 subsegments), the
