@@ -188,7 +188,10 @@ def test_generators():
     # the reference's 3 x 2 x 16 (batches of four patches): 90,003 patches, the
     # last batch holding 3; one patch (a single partial batch); 6 patches (the
     # second batch holding 2)
-    (2000, 4, 30001, 3, 2, 16, 1.0), (3, 2, 1, 3, 2, 16, 1.0), (5, 3, 3, 3, 2, 16, 0.5)])
+    (2000, 4, 30001, 3, 2, 16, 1.0), (3, 2, 1, 3, 2, 16, 1.0), (5, 3, 3, 3, 2, 16, 0.5),
+    # the largest pass the parameters allow: 64 x 64 blends (64 KB of LDS
+    # weights), 64 subsegments
+    (3, 2, 1, 64, 64, 64, 1.0)])
 def test_gpu_matches_oracle(ctx, S, N, T, ninst, nlines, nsub, scale):
     """The pass against the oracle, bit for bit: the batched kernel at the
     reference's tessellation (3 instances x 2 lines x 16 subsegments), the
