@@ -130,7 +130,7 @@ def pmc_ids(db, kernels=None):
     out = ["%-26s %4s %8s %5s %4s %9s " % ("kernel", "id", "wgs", "vgpr", "n", "dur_us") +
            " ".join("%14s" % c for c in names)]
     for k, (means, n, dur, v) in sorted(res.items(), key=lambda kv: -kv[1][0].get("SQ_INSTS_VALU", 0)):
-        if kernels and k[0] not in kernels:
+        if kernels and k[0] not in kernels and base(k[0]) not in kernels:
             continue
         out.append("%-26s %4d %8d %5d %4d %9.1f " % (k[0], k[1], k[2], v, n, dur) +
                    " ".join("%14.6g" % means.get(c, float("nan")) for c in names))

@@ -473,8 +473,8 @@ def main():
                     help="lanes per strand of the single-groom workload (0: the tuned table)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (strands and grooms shard with no "
-              "exchange; see the design notes).  Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (strands and grooms would shard with no "
+              "exchange between GPUs).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import barbu_amd as ba
