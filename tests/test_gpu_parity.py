@@ -404,12 +404,16 @@ def test_skinned_roots(ctx):
     skin.close()
 
 
-def test_c2_full_size_properties(ctx):
-    """262,144 x 32 (the single-groom workload): a few steps checked against the
-    oracle on a strand sample, plus size-independent invariants on all strands:
-    finite state, roots on the rotating emitter, segment lengths where no
-    collider touches, and non-penetration of the last collider."""
-    S, N = 262144, 32
+@pytest.mark.parametrize("S", [262144, 524288])
+def test_c2_full_size_properties(ctx, S):
+    """262,144 x 32 (the single-groom workload), and twice as many strands,
+    whose 537 MB of state take the non-temporal-store form of the wind kernel:
+    a few steps checked against the oracle on a strand sample, plus
+    size-independent invariants on all strands: finite state, roots on the
+    rotating emitter, segment lengths where no collider touches, and
+    non-penetration of the last collider."""
+    N = 32
+    print("c2 properties S=%d" % S, flush=True)
     pos, vel, roots = ba.sphere_groom(S, N, radius=1.0, length=0.2, seed=2024)
     params = ba.make_params(force_scale=1.0, damping=0.9, length_scale=1.0, wind_amplitude=2.0,
                             wind_frequency=2.0, wind_scroll=(0.4, 0.0, 0.2), wind_seed=7)
