@@ -147,16 +147,27 @@ __global__ __launch_bounds__(256) void ftl_expand_kernel(ExpandArgs A) {
 // The same pass with more stores per load round trip: one wave per workgroup
 // takes a batch of B patches.  It loads the batch's triangles, then their
 // guide points and tangents (two round trips, as a thread of
-// ftl_expand_kernel makes for its one patch, but one load per lane instead
-// of thirteen), into LDS, and then every lane blends and stores B * NV / 64
-// vertices of the batch for every (instance, line): 12 stores per lane at the
-// reference's tessellation with B = 4, against 6, with no load between them.
-// NV = 2 nsub vertices per isoline and NC = instances x lines blends are
-// fixed, so the loops unroll.  Same operations per output vertex as
-// ftl_expand_kernel.  (Measured on the 100k-triangle scalp: B = 2, 4, 8, 16
-// ran 0.182, 0.160, 0.177, 0.175 ms.)
-constexpr int kExpandBatch = 4;
+// ftl_expand_kernel makes for its one patch, but at most six loads per lane
+// instead of thirteen), into LDS, and then every lane evaluates B * NV / 64
+// vertex positions of the batch and blends and stores each for every
+// (instance, line), with no load in between: 12 stores per lane at the
+// reference's tessellation (B = 4), against 6.  NV = 2 nsub positions per
+// isoline is fixed per instantiation; the blend count (instances x lines) is
+// a compile-time 6 at the reference's 3 x 2 x 16, so that loop unrolls, and
+// a runtime count otherwise, read four weights at a time from LDS ahead of
+// their stores (a wait on each weight before its store cost up to 8%).  Same operations per output vertex as
+// ftl_expand_kernel.  (Measured at the reference's tessellation on the
+// 100k-triangle scalp: B = 2, 4, 8, 16 ran 0.182, 0.160, 0.177, 0.175 ms.)
 constexpr int kGuideF4 = 12;  // per patch: 3 guides x (point j, point j+1, tangent j, tangent j+1)
+
+// Patches per batch for NV positions per isoline: the smallest multiple of 4
+// whose B * NV positions fill whole rows of 64 lanes (4 at 16 subsegments,
+// at most 32).
+constexpr int expand_batch_of(int nv) {
+  int b = 4;
+  while ((b * nv) % 64 != 0) b += 4;
+  return b;
+}
 
 __device__ __forceinline__ void expand_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -164,8 +175,9 @@ __device__ __forceinline__ void expand_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Blends and stores batch b (patches b * B ...) from its guides in
-// G (LDS).  GUARD: the pass's last batch, whose patches may end early.
+// Blends and stores batch b (patches b * B ...) from its guides in G0 (LDS)
+// and the blend weights in wts (LDS).  GUARD: the pass's last batch, whose
+// patches may end early.  NC > 0: the blend count, at compile time.
 template <int NV, int NC, bool GUARD, int B>
 __device__ __forceinline__ void expand_batch(const ExpandArgs& A, const float4* G0, const float4* wts, int b,
                                              int npatch, int lane) {
@@ -193,14 +205,37 @@ __device__ __forceinline__ void expand_batch(const ExpandArgs& A, const float4* 
       }
       const float w = rel_pos(seg, u, fpts);
       const long long iso = ((long long)tri * nseg + seg) * nlines;
+      auto blend = [&](const float4 cw) {
+        return make_float4(__builtin_fmaf(cw.z, g[2].x, __builtin_fmaf(cw.y, g[1].x, cw.x * g[0].x)),
+                           __builtin_fmaf(cw.z, g[2].y, __builtin_fmaf(cw.y, g[1].y, cw.x * g[0].y)),
+                           __builtin_fmaf(cw.z, g[2].z, __builtin_fmaf(cw.y, g[1].z, cw.x * g[0].z)), w);
+      };
+      if constexpr (NC > 0) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const int inst = c / nlines, line = c % nlines;
-        const float4 cw = wts[c];
-        const float4 o = make_float4(__builtin_fmaf(cw.z, g[2].x, __builtin_fmaf(cw.y, g[1].x, cw.x * g[0].x)),
-                                     __builtin_fmaf(cw.z, g[2].y, __builtin_fmaf(cw.y, g[1].y, cw.x * g[0].y)),
-                                     __builtin_fmaf(cw.z, g[2].z, __builtin_fmaf(cw.y, g[1].z, cw.x * g[0].z)), w);
-        out[(iso + inst * inst_stride + line) * NV + v] = o;
+        for (int c = 0; c < NC; ++c) {
+          const int inst = c / nlines, line = c % nlines;
+          out[(iso + inst * inst_stride + line) * NV + v] = blend(wts[c]);
+        }
+      } else {
+        // (instance, line) in the capture order, four blends at a time: their
+        // weights are read from LDS together, ahead of the four stores
+        const int nc = A.ninst * nlines;
+        int inst = 0, line = 0;
+        for (int c0 = 0; c0 < nc; c0 += 4) {
+          float4 cw[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) cw[k] = wts[min(c0 + k, nc - 1)];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            if (c0 + k < nc) {
+              out[(iso + inst * inst_stride + line) * NV + v] = blend(cw[k]);
+              if (++line == nlines) {
+                line = 0;
+                ++inst;
+              }
+            }
+          }
+        }
       }
     }
   }
@@ -208,11 +243,12 @@ __device__ __forceinline__ void expand_batch(const ExpandArgs& A, const float4* 
 
 template <int NV, int NC, int B>
 __global__ __launch_bounds__(64) void ftl_expand_batched_kernel(ExpandArgs A) {
-  static_assert(64 % NV == 0 && (B * NV) % 64 == 0 && B <= 64, "whole patches per wave-wide step");
+  static_assert(NV % 2 == 0 && (B * NV) % 64 == 0 && B <= 64, "whole patches per wave-wide step");
   constexpr int kGuides = B * kGuideF4;
   extern __shared__ float4 sm[];
-  float4* wts = sm;                     // [NC]
-  float4* gd = sm + NC;                 // [B * kGuideF4]
+  const int nc = NC > 0 ? NC : A.ninst * A.nlines;
+  float4* wts = sm;                     // [nc]
+  float4* gd = sm + nc;                 // [B * kGuideF4]
   int4* td = (int4*)(gd + kGuides);     // [B]
   const int lane = (int)threadIdx.x;
   const int nseg = A.nseg, nlines = A.nlines;
@@ -222,7 +258,7 @@ __global__ __launch_bounds__(64) void ftl_expand_batched_kernel(ExpandArgs A) {
   // lanes past a run repeat another lane's load.
   const int ti = lane % B;
   const int4 tv0 = A.tris[min(b * B + ti, last) / nseg];
-  for (int c = lane; c < NC; c += 64) {
+  for (int c = lane; c < nc; c += 64) {
     const float3 bw = bary(A.rnd, c / nlines, c % nlines, nlines);
     wts[c] = make_float4(bw.x, bw.y, bw.z, 0.0f);
   }
@@ -250,6 +286,42 @@ __global__ __launch_bounds__(64) void ftl_expand_batched_kernel(ExpandArgs A) {
     expand_batch<NV, NC, false, B>(A, gd, wts, b, npatch, lane);
   else
     expand_batch<NV, NC, true, B>(A, gd, wts, b, npatch, lane);
+}
+
+// The batched pass at nsub = NSUB subsegments (nc blends).
+template <int NSUB>
+void launch_expand_batched(const ExpandArgs& A, int nc, hipStream_t stream) {
+  constexpr int NV = 2 * NSUB, B = expand_batch_of(NV);
+  static_assert(B <= 8 && B * NV / 64 <= 3, "the subsegment counts the batched pass takes");
+  const int npatch = A.ntris * A.nseg;
+  const int nbatch = (npatch + B - 1) / B;
+  const size_t lds = (size_t)nc * sizeof(float4) + (size_t)B * kGuideF4 * sizeof(float4) + B * sizeof(int4);
+  if constexpr (NSUB == 16) {
+    if (nc == 6) {
+      hipLaunchKernelGGL((ftl_expand_batched_kernel<NV, 6, B>), dim3(nbatch), dim3(64), lds, stream, A);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((ftl_expand_batched_kernel<NV, 0, B>), dim3(nbatch), dim3(64), lds, stream, A);
+}
+
+// Launches the batched pass where it measured faster than ftl_expand_kernel
+// on the 100k-triangle scalp (0 to -33% per pass): at most 10 blends and 8,
+// 12 or 16 subsegments (batches of 4 or 8 patches), the reference's 3 x 2 x
+// 16 among them.  Otherwise it launches nothing and returns false.  More
+// blends per position already amortise the per-vertex kernel's loads (12 or
+// 16 blends: 0 to +5% batched); fewer subsegments need batches of 16 or 32
+// patches, which leave too few waves (3 x 2 x 3: +12%, 2 x 3 x 6: +8%,
+// 3 x 3 x 7: +22%, 3 x 2 x 15: +13%), and 4 subsegments gain nothing
+// (3 x 2 x 4: +2%).
+bool launch_expand_batched_if_faster(const ExpandArgs& A, int nsub, int nc, hipStream_t st) {
+  if (nc > 10) return false;
+  switch (nsub) {
+    case 8: launch_expand_batched<8>(A, nc, st); return true;
+    case 12: launch_expand_batched<12>(A, nc, st); return true;
+    case 16: launch_expand_batched<16>(A, nc, st); return true;
+    default: return false;
+  }
 }
 
 bool params_ok(const ftl_expand_params* p) {
@@ -416,15 +488,8 @@ extern "C" ftl_status ftl_expand_run(ftl_expand* e, const ftl_expand_params* p, 
   A.nthreads = (int)nthreads;
   A.scale = p->tangent_scale;
   const size_t lds = (size_t)p->ninstances * p->nlines * sizeof(float4);
-  if (p->nsubsegments == 16 && p->ninstances * p->nlines == 6) {
-    // the reference's tessellation (3 instances x 2 lines x 16 subsegments,
-    // hair.h): one wave per batch of kExpandBatch patches
-    const int npatch = e->ntris * (e->npoints - 1);
-    const int nbatch = (npatch + kExpandBatch - 1) / kExpandBatch;
-    const size_t blds = lds + (size_t)kExpandBatch * kGuideF4 * sizeof(float4) + kExpandBatch * sizeof(int4);
-    hipLaunchKernelGGL((ftl_expand_batched_kernel<32, 6, kExpandBatch>), dim3(nbatch), dim3(64), blds,
-                       (hipStream_t)stream, A);
-  } else {
+  const int nc = p->ninstances * p->nlines;
+  if (!launch_expand_batched_if_faster(A, p->nsubsegments, nc, (hipStream_t)stream)) {
     const int blocks = (int)((nthreads + 255) / 256);
     hipLaunchKernelGGL(ftl_expand_kernel, dim3(blocks), dim3(256), lds, (hipStream_t)stream, A);
   }

@@ -211,6 +211,32 @@ def test_gpu_matches_oracle(ctx, S, N, T, ninst, nlines, nsub, scale):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("ninst,nlines,nsub", [
+    # the batched pass with the blend count at run time: batches of 4 and 8
+    # patches, 1 to 3 positions per lane, the last batch partial here
+    (1, 1, 16), (2, 5, 16), (3, 1, 8), (1, 8, 8), (1, 4, 12), (3, 2, 12), (2, 5, 12),
+    # the per-vertex kernel: more blends, other subsegment counts, and past
+    # the reference's settings (1 to 16 each)
+    (16, 16, 16), (4, 4, 7), (5, 3, 12), (7, 2, 15), (3, 2, 15), (5, 2, 14), (2, 2, 10), (2, 5, 3),
+    (2, 3, 6), (1, 3, 4), (3, 3, 7), (1, 2, 2), (16, 16, 1), (3, 2, 17), (17, 16, 2)])
+def test_gpu_tessellation_settings(ctx, ninst, nlines, nsub):
+    """Settings across the reference's ranges, through whichever form the
+    launch takes; against the oracle, bit for bit."""
+    import barbu_amd as ba
+    S, N, T = 200, 6, 300
+    print("expand settings inst=%d lines=%d sub=%d" % (ninst, nlines, nsub), flush=True)
+    pos, tan, tris, rnd = scene(S, N, T, R=4096, seed=ninst * 100 + nlines * 10 + nsub)
+    g = ba.Groom(ctx, pos)
+    e = ba.Expand(ctx, S, N, tris, tan, rnd)
+    got = e.run_groom(g, ba.expand_params(ninst, nlines, nsub, 1.1))
+    want = orc.expand_f32(pos, tan, tris, rnd, ninst, nlines, nsub, 1.1)
+    from tests.helpers import assert_bitexact
+    assert_bitexact(got, want, "expansion %d x %d x %d" % (ninst, nlines, nsub))
+    e.close()
+    g.close()
+
+
+@pytest.mark.gpu
 def test_gpu_errors(ctx):
     import barbu_amd as ba
     pos, tan, tris, rnd = scene(R=64)
