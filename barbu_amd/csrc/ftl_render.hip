@@ -11,6 +11,10 @@
 // between those -- and writes one float4 per blend.  Consecutive threads take
 // consecutive vertices of an isoline, so every store of a wave writes runs of
 // 2 * nsub float4 (512 B at the reference's 16 subsegments): whole lines.
+// At the settings where it measured faster (the reference's among them), a
+// second form batches patches per wave instead: their guides are loaded once
+// into LDS and each lane stores several vertices' blends
+// (ftl_expand_batched_kernel).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
